@@ -1,0 +1,911 @@
+// CPU restatement of the reference's merge-scoring path -- TEST INFRASTRUCTURE.
+// Only tests/, the smoke check and the benchmark's CPU-baseline leg use this
+// file; the product never links it.  Line citations are to the reference
+// repository (src/..., include/...).
+#include "oracle.hpp"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <limits>
+#include <map>
+#include <numeric>
+#include <sstream>
+#include <stdexcept>
+#include <unordered_set>
+
+namespace oracle {
+
+namespace {
+
+int64_t Checked(__int128 v, const char* what) {
+  if (v < 0 || v > std::numeric_limits<int64_t>::max()) throw std::overflow_error(what);
+  return static_cast<int64_t>(v);
+}
+
+void AddChecked(int64_t* total, int64_t v, const char* what) {
+  if (v < 0 || *total > std::numeric_limits<int64_t>::max() - v) throw std::overflow_error(what);
+  *total += v;
+}
+
+uint64_t PairKey(int a, int b) {
+  const uint32_t lo = static_cast<uint32_t>(std::min(a, b));
+  const uint32_t hi = static_cast<uint32_t>(std::max(a, b));
+  return (static_cast<uint64_t>(lo) << 32U) | hi;
+}
+
+// EncodeCostForPair, sweg.cpp:1423-1433: unchecked min(e, 1 + cap - e); may be
+// negative for the doubled self entry.
+int64_t EncodeCost(int rep_u, int64_t su, int rep_x, int64_t sx, int64_t e) {
+  const int64_t cap = rep_u == rep_x ? su * (su - 1) / 2 : su * sx;
+  return std::min(e, 1 + cap - e);
+}
+
+}  // namespace
+
+uint64_t SplitMix64(uint64_t x) {  // sweg.cpp:2290-2295
+  x += 0x9e3779b97f4a7c15ULL;
+  x = (x ^ (x >> 30U)) * 0xbf58476d1ce4e5b9ULL;
+  x = (x ^ (x >> 27U)) * 0x94d049bb133111ebULL;
+  return x ^ (x >> 31U);
+}
+
+// graph_io.cpp:12-106: '#' and blank lines skipped; at least two integers per
+// line; negative, > INT_MAX and self-loop ids rejected; both arcs added; rows
+// sorted; duplicates kept.
+Graph LoadEdgeList(const std::string& path) {
+  std::ifstream in(path);
+  if (!in) throw std::runtime_error("Failed to open file: " + path);
+  std::vector<std::pair<int, int>> arcs;
+  int64_t max_node = -1, raw = 0, line_no = 0;
+  std::string line;
+  while (std::getline(in, line)) {
+    ++line_no;
+    const auto first = line.find_first_not_of(" \t\r");
+    if (first == std::string::npos || line[first] == '#') continue;
+    std::istringstream iss(line);
+    int64_t u = 0, v = 0;
+    if (!(iss >> u >> v)) {
+      throw std::runtime_error("Malformed edge list line " + std::to_string(line_no) +
+                               ": expected at least two integers 'u v'.");
+    }
+    if (u < 0 || v < 0) throw std::runtime_error("Negative node id at line " + std::to_string(line_no) + ".");
+    if (u > std::numeric_limits<int>::max() || v > std::numeric_limits<int>::max()) {
+      throw std::runtime_error("Node id exceeds int range at line " + std::to_string(line_no) + ".");
+    }
+    if (u == v) throw std::runtime_error("Self-loop is not supported at line " + std::to_string(line_no) + ".");
+    ++raw;
+    arcs.emplace_back(static_cast<int>(u), static_cast<int>(v));
+    arcs.emplace_back(static_cast<int>(v), static_cast<int>(u));
+    max_node = std::max({max_node, u, v});
+  }
+  if (max_node + 1 <= 0) throw std::runtime_error("Invalid node count.");
+  Graph g = FromEdges(static_cast<int>(max_node + 1), {});
+  g.input_edges_raw = raw;
+  g.m = static_cast<int64_t>(arcs.size());
+  g.row_ptr.assign(static_cast<size_t>(g.n) + 1, 0);
+  for (const auto& a : arcs) g.row_ptr[static_cast<size_t>(a.first) + 1]++;
+  for (int i = 0; i < g.n; ++i) g.row_ptr[static_cast<size_t>(i) + 1] += g.row_ptr[static_cast<size_t>(i)];
+  g.col_idx.assign(static_cast<size_t>(g.m), 0);
+  std::vector<int64_t> cur(g.row_ptr.begin(), g.row_ptr.end() - 1);
+  for (const auto& a : arcs) g.col_idx[static_cast<size_t>(cur[static_cast<size_t>(a.first)]++)] = a.second;
+  for (int v = 0; v < g.n; ++v) {
+    std::sort(g.col_idx.begin() + g.row_ptr[static_cast<size_t>(v)],
+              g.col_idx.begin() + g.row_ptr[static_cast<size_t>(v) + 1]);
+  }
+  return g;
+}
+
+Graph FromEdges(int n, const std::vector<std::pair<int, int>>& edges) {
+  Graph g;
+  g.n = n;
+  g.input_edges_raw = static_cast<int64_t>(edges.size());
+  g.m = 2 * g.input_edges_raw;
+  g.row_ptr.assign(static_cast<size_t>(n) + 1, 0);
+  for (const auto& e : edges) {
+    g.row_ptr[static_cast<size_t>(e.first) + 1]++;
+    g.row_ptr[static_cast<size_t>(e.second) + 1]++;
+  }
+  for (int i = 0; i < n; ++i) g.row_ptr[static_cast<size_t>(i) + 1] += g.row_ptr[static_cast<size_t>(i)];
+  g.col_idx.assign(static_cast<size_t>(g.m), 0);
+  std::vector<int64_t> cur(g.row_ptr.begin(), g.row_ptr.end() - 1);
+  for (const auto& e : edges) {
+    g.col_idx[static_cast<size_t>(cur[static_cast<size_t>(e.first)]++)] = e.second;
+    g.col_idx[static_cast<size_t>(cur[static_cast<size_t>(e.second)]++)] = e.first;
+  }
+  for (int v = 0; v < n; ++v) {
+    std::sort(g.col_idx.begin() + g.row_ptr[static_cast<size_t>(v)],
+              g.col_idx.begin() + g.row_ptr[static_cast<size_t>(v) + 1]);
+  }
+  return g;
+}
+
+// cost_oracle.cpp:93-110.
+int64_t BlockCost(int64_t sa, int64_t sb, int64_t e, bool internal) {
+  if (sa < 0 || sb < 0 || e < 0) throw std::invalid_argument("block sizes and edge count must be non-negative");
+  if (internal && sa != sb) throw std::invalid_argument("internal block sizes must match");
+  const __int128 cap_wide = internal ? static_cast<__int128>(sa) * (sa - 1) / 2 : static_cast<__int128>(sa) * sb;
+  const int64_t cap = Checked(cap_wide, "block capacity overflow");
+  if (e > cap) throw std::invalid_argument("edge count exceeds block capacity");
+  return std::min(e, 1 + cap - e);
+}
+
+namespace {
+
+// cost_oracle.cpp:38-84: compact labels, block sizes and per-block edge counts.
+struct LabelState {
+  std::vector<int64_t> sizes;
+  std::map<int, int> compact;
+  std::unordered_map<uint64_t, int64_t> edges;
+};
+
+LabelState BuildLabelState(const Graph& g, const std::vector<int>& labels) {
+  if (labels.size() != static_cast<size_t>(g.n)) throw std::invalid_argument("partition labels must match graph.n");
+  LabelState s;
+  std::vector<int> c(labels.size());
+  for (int v = 0; v < g.n; ++v) {
+    const int label = labels[static_cast<size_t>(v)];
+    if (label < 0) throw std::invalid_argument("partition labels must be non-negative");
+    auto it = s.compact.emplace(label, static_cast<int>(s.sizes.size())).first;
+    if (it->second == static_cast<int>(s.sizes.size())) s.sizes.push_back(0);
+    c[static_cast<size_t>(v)] = it->second;
+    ++s.sizes[static_cast<size_t>(it->second)];
+  }
+  for (int u = 0; u < g.n; ++u) {
+    for (int64_t k = g.row_ptr[static_cast<size_t>(u)]; k < g.row_ptr[static_cast<size_t>(u) + 1]; ++k) {
+      const int v = g.col_idx[static_cast<size_t>(k)];
+      if (u == v) throw std::invalid_argument("CostOracle does not support self-loops");
+      if (u < v) ++s.edges[PairKey(c[static_cast<size_t>(u)], c[static_cast<size_t>(v)])];
+    }
+  }
+  return s;
+}
+
+int64_t LabelEdges(const LabelState& s, int a, int b) {
+  const auto it = s.edges.find(PairKey(a, b));
+  return it == s.edges.end() ? 0 : it->second;
+}
+
+}  // namespace
+
+int64_t ExactPartitionCost(const Graph& g, const std::vector<int>& labels) {  // cost_oracle.cpp:112-124
+  const LabelState s = BuildLabelState(g, labels);
+  int64_t total = 0;
+  for (const auto& [key, e] : s.edges) {
+    const int a = static_cast<int>(key >> 32U), b = static_cast<int>(key & 0xffffffffU);
+    AddChecked(&total, BlockCost(s.sizes[static_cast<size_t>(a)], s.sizes[static_cast<size_t>(b)], e, a == b),
+               "partition cost overflow");
+  }
+  return total;
+}
+
+int64_t FullMergeGain(const Graph& g, const std::vector<int>& labels, int la, int lb) {  // cost_oracle.cpp:126-168
+  if (la == lb) throw std::invalid_argument("merge endpoints must have different labels");
+  const LabelState s = BuildLabelState(g, labels);
+  const int a = s.compact.at(la), b = s.compact.at(lb);
+  const int64_t sa = s.sizes[static_cast<size_t>(a)], sb = s.sizes[static_cast<size_t>(b)], sw = sa + sb;
+  int64_t before = 0, after = 0;
+  const int64_t internal = LabelEdges(s, a, a) + LabelEdges(s, b, b) + LabelEdges(s, a, b);
+  AddChecked(&before, BlockCost(sa, sa, LabelEdges(s, a, a), true), "gain");
+  AddChecked(&before, BlockCost(sb, sb, LabelEdges(s, b, b), true), "gain");
+  AddChecked(&before, BlockCost(sa, sb, LabelEdges(s, a, b), false), "gain");
+  AddChecked(&after, BlockCost(sw, sw, internal, true), "gain");
+  for (int x = 0; x < static_cast<int>(s.sizes.size()); ++x) {
+    if (x == a || x == b) continue;
+    const int64_t sx = s.sizes[static_cast<size_t>(x)];
+    const int64_t ea = LabelEdges(s, a, x), eb = LabelEdges(s, b, x);
+    AddChecked(&before, BlockCost(sa, sx, ea, false), "gain");
+    AddChecked(&before, BlockCost(sb, sx, eb, false), "gain");
+    AddChecked(&after, BlockCost(sw, sx, ea + eb, false), "gain");
+  }
+  return before - after;
+}
+
+// ---------------------------------------------------------------- Quotient
+
+void Quotient::Rebuild(const Graph& g, const std::vector<int>& labels) {  // quotient_graph.cpp:80-145
+  const size_t n = static_cast<size_t>(g.n);
+  if (labels.size() != n) throw std::invalid_argument("invalid graph or partition shape");
+  sizes_.assign(n, 0);
+  internal_.assign(n, 0);
+  rows_.assign(n, {});
+  active_.assign(n, 0);
+  for (int v = 0; v < g.n; ++v) {
+    const int label = labels[static_cast<size_t>(v)];
+    if (label < 0 || label >= g.n) throw std::invalid_argument("partition label is out of range");
+    active_[static_cast<size_t>(label)] = 1;
+    ++sizes_[static_cast<size_t>(label)];
+  }
+  for (int u = 0; u < g.n; ++u) {
+    for (int64_t k = g.row_ptr[static_cast<size_t>(u)]; k < g.row_ptr[static_cast<size_t>(u) + 1]; ++k) {
+      const int v = g.col_idx[static_cast<size_t>(k)];
+      if (u == v) throw std::invalid_argument("QuotientGraph requires a loop-free graph with valid neighbors");
+      if (u >= v) continue;
+      const int a = labels[static_cast<size_t>(u)], b = labels[static_cast<size_t>(v)];
+      if (a == b) {
+        ++internal_[static_cast<size_t>(a)];
+      } else {
+        rows_[static_cast<size_t>(a)].emplace_back(b, 1);
+        rows_[static_cast<size_t>(b)].emplace_back(a, 1);
+      }
+    }
+  }
+  for (Row& row : rows_) {
+    std::sort(row.begin(), row.end());
+    size_t out = 0;
+    for (const auto& e : row) {
+      if (out > 0 && row[out - 1].first == e.first) {
+        row[out - 1].second += e.second;
+      } else {
+        row[out++] = e;
+      }
+    }
+    row.resize(out);
+  }
+}
+
+// quotient_graph.cpp:340-462: merge-join of the two sorted rows.  For every
+// union neighbour x not in {a, b}: before += BC(sa,sx,ea) + BC(sb,sx,eb) when x
+// is in both rows, else BC(side,sx,e); after += BC(sa+sb,sx,ea+eb).  Internal
+// terms follow; the a-b count must agree from both sides.
+GainResult Quotient::Gain(int a, int b, uint64_t* raw_entries) const {
+  if (!Active(a) || !Active(b)) throw std::invalid_argument("quotient supernode is inactive");
+  if (a == b) throw std::invalid_argument("merge endpoints must differ");
+  const int64_t sa = Size(a), sb = Size(b);
+  const int64_t sw = Checked(static_cast<__int128>(sa) + sb, "quotient block capacity overflow");
+  const Row& ra = RowOf(a);
+  const Row& rb = RowOf(b);
+  int64_t before = 0, after = 0, eab_a = 0, eab_b = 0;
+  size_t ia = 0, ib = 0;
+  while (ia < ra.size() || ib < rb.size()) {
+    int x;
+    int64_t ea = 0, eb = 0;
+    bool overlap = false;
+    if (ib >= rb.size() || (ia < ra.size() && ra[ia].first < rb[ib].first)) {
+      x = ra[ia].first;
+      ea = ra[ia++].second;
+    } else if (ia >= ra.size() || rb[ib].first < ra[ia].first) {
+      x = rb[ib].first;
+      eb = rb[ib++].second;
+    } else {
+      x = ra[ia].first;
+      ea = ra[ia++].second;
+      eb = rb[ib++].second;
+      overlap = true;
+    }
+    if (x == b) {
+      eab_a = ea;
+      continue;
+    }
+    if (x == a) {
+      eab_b = eb;
+      continue;
+    }
+    const int64_t sx = Size(x);
+    if (overlap) {
+      AddChecked(&before, BlockCost(sa, sx, ea, false), "merge gain cost overflow");
+      AddChecked(&before, BlockCost(sb, sx, eb, false), "merge gain cost overflow");
+    } else {
+      AddChecked(&before, ea != 0 ? BlockCost(sa, sx, ea + eb, false) : BlockCost(sb, sx, ea + eb, false),
+                 "merge gain cost overflow");
+    }
+    AddChecked(&after, BlockCost(sw, sx, ea + eb, false), "merge gain cost overflow");
+  }
+  if (eab_a != eab_b) throw std::runtime_error("asymmetric quotient edge count during scoring");
+  AddChecked(&before, BlockCost(sa, sa, Internal(a), true), "merge gain cost overflow");
+  AddChecked(&before, BlockCost(sb, sb, Internal(b), true), "merge gain cost overflow");
+  AddChecked(&before, BlockCost(sa, sb, eab_a, false), "merge gain cost overflow");
+  AddChecked(&after, BlockCost(sw, sw, Internal(a) + Internal(b) + eab_a, true), "merge gain cost overflow");
+  if (raw_entries != nullptr) *raw_entries += ra.size() + rb.size();
+  return {before - after, before};
+}
+
+namespace {
+
+void SetCount(Quotient::Row& row, int nbr, int64_t count) {  // quotient_graph.cpp:193-217
+  auto it = std::lower_bound(row.begin(), row.end(), nbr,
+                             [](const auto& e, int v) { return e.first < v; });
+  if (it != row.end() && it->first == nbr) {
+    if (count == 0) {
+      row.erase(it);
+    } else {
+      it->second = count;
+    }
+  } else if (count != 0) {
+    row.insert(it, {nbr, count});
+  }
+}
+
+int64_t GetCount(const Quotient::Row& row, int nbr) {  // quotient_graph.cpp:185-191
+  auto it = std::lower_bound(row.begin(), row.end(), nbr,
+                             [](const auto& e, int v) { return e.first < v; });
+  return it == row.end() || it->first != nbr ? 0 : it->second;
+}
+
+}  // namespace
+
+// quotient_graph.cpp:628-742: keep absorbs remove; the keep row becomes the
+// sorted union; every neighbour of remove re-points its entry to keep.
+void Quotient::Merge(int keep, int remove) {
+  if (!Active(keep) || !Active(remove) || keep == remove) throw std::invalid_argument("bad merge");
+  const size_t k = static_cast<size_t>(keep), r = static_cast<size_t>(remove);
+  const int64_t cross = GetCount(rows_[k], remove);
+  sizes_[k] += sizes_[r];
+  internal_[k] += internal_[r] + cross;
+  SetCount(rows_[k], remove, 0);
+  SetCount(rows_[r], keep, 0);
+  const Row old_keep = rows_[k];
+  const Row removed = rows_[r];
+  Row merged;
+  merged.reserve(old_keep.size() + removed.size());
+  size_t i = 0, j = 0;
+  while (i < old_keep.size() || j < removed.size()) {
+    if (j >= removed.size() || (i < old_keep.size() && old_keep[i].first < removed[j].first)) {
+      merged.push_back(old_keep[i++]);
+    } else if (i >= old_keep.size() || removed[j].first < old_keep[i].first) {
+      merged.push_back(removed[j++]);
+    } else {
+      merged.emplace_back(old_keep[i].first, old_keep[i].second + removed[j].second);
+      ++i;
+      ++j;
+    }
+  }
+  rows_[k] = std::move(merged);
+  for (const auto& [x, c] : removed) {
+    (void)c;
+    const int64_t merged_count = GetCount(rows_[k], x);
+    SetCount(rows_[static_cast<size_t>(x)], remove, 0);
+    SetCount(rows_[static_cast<size_t>(x)], keep, merged_count);
+  }
+  rows_[r].clear();
+  sizes_[r] = 0;
+  internal_[r] = 0;
+  active_[r] = 0;
+}
+
+// quotient_graph.cpp:759-856: order-free merge of endpoint-disjoint pairs.
+void Quotient::MergeBulk(const std::vector<std::pair<int, int>>& pairs) {
+  if (pairs.empty()) return;
+  std::vector<int> dest(active_.size(), -1);
+  for (size_t v = 0; v < active_.size(); ++v)
+    if (active_[v]) dest[v] = static_cast<int>(v);
+  for (const auto& [k, r] : pairs) {
+    if (!Active(k) || !Active(r) || k == r || dest[static_cast<size_t>(k)] != k ||
+        dest[static_cast<size_t>(r)] != r) {
+      throw std::invalid_argument("bulk quotient merge requires endpoint-disjoint active pairs");
+    }
+    dest[static_cast<size_t>(r)] = k;
+  }
+  std::vector<int64_t> ns = sizes_, ni = internal_;
+  std::vector<Row> nrows(active_.size());
+  for (const auto& [k, r] : pairs) {
+    ns[static_cast<size_t>(k)] += sizes_[static_cast<size_t>(r)];
+    ni[static_cast<size_t>(k)] += internal_[static_cast<size_t>(r)];
+    ns[static_cast<size_t>(r)] = 0;
+    ni[static_cast<size_t>(r)] = 0;
+  }
+  for (int a = 0; a < N(); ++a) {
+    if (!Active(a)) continue;
+    for (const auto& [b, c] : rows_[static_cast<size_t>(a)]) {
+      if (a >= b) continue;
+      const int ma = dest[static_cast<size_t>(a)], mb = dest[static_cast<size_t>(b)];
+      if (ma == mb) {
+        ni[static_cast<size_t>(ma)] += c;
+      } else {
+        nrows[static_cast<size_t>(ma)].emplace_back(mb, c);
+        nrows[static_cast<size_t>(mb)].emplace_back(ma, c);
+      }
+    }
+  }
+  for (Row& row : nrows) {
+    std::sort(row.begin(), row.end());
+    size_t out = 0;
+    for (const auto& e : row) {
+      if (out > 0 && row[out - 1].first == e.first) {
+        row[out - 1].second += e.second;
+      } else {
+        row[out++] = e;
+      }
+    }
+    row.resize(out);
+  }
+  for (const auto& [k, r] : pairs) {
+    (void)k;
+    active_[static_cast<size_t>(r)] = 0;
+  }
+  sizes_.swap(ns);
+  internal_.swap(ni);
+  rows_.swap(nrows);
+}
+
+int64_t Quotient::ExactCost() const {  // quotient_graph.cpp:240-257
+  int64_t total = 0;
+  for (int a = 0; a < N(); ++a) {
+    if (!Active(a)) continue;
+    AddChecked(&total, BlockCost(Size(a), Size(a), Internal(a), true), "quotient cost overflow");
+    for (const auto& [b, e] : RowOf(a)) {
+      if (a < b) AddChecked(&total, BlockCost(Size(a), Size(b), e, false), "quotient cost overflow");
+    }
+  }
+  return total;
+}
+
+uint64_t Quotient::Nnz() const {  // quotient_graph.cpp:259-270
+  uint64_t total = 0;
+  for (int a = 0; a < N(); ++a) {
+    if (!Active(a)) continue;
+    total += RowOf(a).size() + (Internal(a) > 0 ? 1 : 0);
+  }
+  return total;
+}
+
+bool Quotient::SameAs(const Quotient& o) const {
+  if (active_ != o.active_) return false;
+  for (int a = 0; a < N(); ++a) {
+    if (!Active(a)) continue;
+    if (sizes_[static_cast<size_t>(a)] != o.sizes_[static_cast<size_t>(a)] ||
+        internal_[static_cast<size_t>(a)] != o.internal_[static_cast<size_t>(a)] ||
+        rows_[static_cast<size_t>(a)] != o.rows_[static_cast<size_t>(a)]) {
+      return false;
+    }
+  }
+  return true;
+}
+
+// ---------------------------------------------------------------- Sweg
+
+Sweg::Sweg(const Graph& g, const Options& opt) : g_(&g), opt_(opt), n_(g.n) {  // sweg.cpp:102-194
+  opt_.group_batch_size = std::max(1, opt_.group_batch_size);
+  if (opt_.divide_hash_dims <= 0) opt_.divide_hash_dims = 16;
+  if (opt_.divide_max_group <= 0) opt_.divide_max_group = 512;
+  I_.resize(static_cast<size_t>(n_));
+  J_.assign(static_cast<size_t>(n_), -1);
+  S_.resize(static_cast<size_t>(n_));
+  sizes_by_rep_.assign(static_cast<size_t>(n_), 1);
+  std::vector<int> labels(static_cast<size_t>(n_));
+  for (int i = 0; i < n_; ++i) {
+    I_[static_cast<size_t>(i)] = i;
+    S_[static_cast<size_t>(i)] = i;
+    labels[static_cast<size_t>(i)] = i;
+  }
+  q_.Rebuild(g, labels);
+}
+
+uint64_t Sweg::HashRank64(int node, int dim, int iter) const {  // sweg.cpp:731-737
+  uint64_t x = opt_.seed;
+  x ^= 0x9e3779b97f4a7c15ULL * static_cast<uint64_t>(node + 1);
+  x ^= 0xbf58476d1ce4e5b9ULL * static_cast<uint64_t>(dim + 1);
+  x ^= 0x94d049bb133111ebULL * static_cast<uint64_t>(iter + 1);
+  return SplitMix64(x);
+}
+
+uint64_t Sweg::DimRank(int dim, int iter) const {  // sweg.cpp:739-744
+  uint64_t x = opt_.seed;
+  x ^= 0x9e3779b97f4a7c15ULL * static_cast<uint64_t>(dim + 1);
+  x ^= 0xbf58476d1ce4e5b9ULL * static_cast<uint64_t>(iter + 1);
+  return SplitMix64(x);
+}
+
+uint64_t Sweg::FallbackRank(int rep, int iter) const {  // sweg.cpp:746-752
+  uint64_t x = opt_.seed;
+  x ^= 0x9e3779b97f4a7c15ULL * static_cast<uint64_t>(rep + 1);
+  x ^= 0xbf58476d1ce4e5b9ULL * static_cast<uint64_t>(iter + 1);
+  x ^= 0xd6e8feb86659fd93ULL;
+  return SplitMix64(x);
+}
+
+// sweg.cpp:754-784: min over the members' closed original neighbourhoods.
+uint64_t Sweg::SupernodeMinHash(int rep, int dim, int iter) {
+  const uint64_t key = (static_cast<uint64_t>(static_cast<uint32_t>(rep)) << 32U) | static_cast<uint32_t>(dim);
+  auto it = minhash_cache_.find(key);
+  if (it != minhash_cache_.end()) return it->second;
+  uint64_t best = std::numeric_limits<uint64_t>::max();
+  for (int v = I_[static_cast<size_t>(rep)]; v != -1; v = J_[static_cast<size_t>(v)]) {
+    uint64_t shingle = HashRank64(v, dim, iter);
+    for (int64_t k = g_->row_ptr[static_cast<size_t>(v)]; k < g_->row_ptr[static_cast<size_t>(v) + 1]; ++k) {
+      shingle = std::min(shingle, HashRank64(g_->col_idx[static_cast<size_t>(k)], dim, iter));
+    }
+    best = std::min(best, shingle);
+  }
+  minhash_cache_.emplace(key, best);
+  return best;
+}
+
+void Sweg::FallbackSplit(int l, int r, int iter) {  // sweg.cpp:786-810
+  if (l >= r) return;
+  stats_.divide_fallback_splits += 1;
+  std::stable_sort(active_reps_.begin() + l, active_reps_.begin() + r, [&](int a, int b) {
+    const uint64_t ha = FallbackRank(a, iter), hb = FallbackRank(b, iter);
+    return ha != hb ? ha < hb : a < b;
+  });
+  for (int s = l; s < r; s += opt_.divide_max_group) {
+    const int e = std::min(s + opt_.divide_max_group, r);
+    groups_.emplace_back(s, e - s);
+    stats_.divide_max_group_size = std::max<int64_t>(stats_.divide_max_group_size, e - s);
+  }
+}
+
+void Sweg::Split(int l, int r, int depth, int iter) {  // sweg.cpp:812-851
+  if (l >= r) return;
+  if (r - l <= opt_.divide_max_group) {
+    groups_.emplace_back(l, r - l);
+    stats_.divide_max_group_size = std::max<int64_t>(stats_.divide_max_group_size, r - l);
+    return;
+  }
+  if (depth >= opt_.divide_hash_dims) {
+    FallbackSplit(l, r, iter);
+    return;
+  }
+  const int dim = dim_order_[static_cast<size_t>(depth)];
+  std::stable_sort(active_reps_.begin() + l, active_reps_.begin() + r, [&](int a, int b) {
+    const uint64_t ha = SupernodeMinHash(a, dim, iter), hb = SupernodeMinHash(b, dim, iter);
+    return ha != hb ? ha < hb : a < b;
+  });
+  int begin = l;
+  while (begin < r) {
+    const uint64_t h = SupernodeMinHash(active_reps_[static_cast<size_t>(begin)], dim, iter);
+    int end = begin + 1;
+    while (end < r && SupernodeMinHash(active_reps_[static_cast<size_t>(end)], dim, iter) == h) ++end;
+    Split(begin, end, depth + 1, iter);
+    begin = end;
+  }
+}
+
+void Sweg::Divide(int iter) {  // sweg.cpp:853-890
+  active_reps_.clear();
+  groups_.clear();
+  minhash_cache_.clear();
+  for (int rep = 0; rep < n_; ++rep)
+    if (I_[static_cast<size_t>(rep)] != -1) active_reps_.push_back(rep);
+  dim_order_.resize(static_cast<size_t>(opt_.divide_hash_dims));
+  std::iota(dim_order_.begin(), dim_order_.end(), 0);
+  std::stable_sort(dim_order_.begin(), dim_order_.end(), [&](int a, int b) {
+    const uint64_t ra = DimRank(a, iter), rb = DimRank(b, iter);
+    return ra != rb ? ra < rb : a < b;
+  });
+  Split(0, static_cast<int>(active_reps_.size()), 0, iter);
+}
+
+// sweg.cpp:1598-1999, persistent state + legacy candidate index.
+std::vector<CandidatePair> Sweg::PrepareGroup(const std::vector<int>& q, uint64_t* proxy_examined) const {
+  const int g = static_cast<int>(q.size());
+  std::vector<CandidatePair> out;
+  if (g < 2) return out;
+  const int top_k = std::max(1, std::min(opt_.top_k, g - 1));  // :1620
+  // Row views with the doubled self entry (quotient_graph.cpp:596-616).
+  std::vector<Quotient::Row> view(static_cast<size_t>(g));
+  std::vector<int64_t> size(static_cast<size_t>(g));
+  for (int i = 0; i < g; ++i) {
+    const int rep = q[static_cast<size_t>(i)];
+    size[static_cast<size_t>(i)] = q_.Size(rep);
+    view[static_cast<size_t>(i)] = q_.RowOf(rep);
+    if (q_.Internal(rep) > 0) {
+      auto& row = view[static_cast<size_t>(i)];
+      auto it = std::lower_bound(row.begin(), row.end(), rep, [](const auto& e, int v) { return e.first < v; });
+      row.insert(it, {rep, 2 * q_.Internal(rep)});
+    }
+  }
+  std::unordered_map<int, int> index_by_rep;  // :1816-1824
+  for (int i = 0; i < g; ++i) index_by_rep.emplace(q[static_cast<size_t>(i)], i);
+
+  struct TargetEntry {
+    int i;
+    int64_t e;
+  };
+  std::unordered_map<int, std::vector<TargetEntry>> buckets;  // :1826-1850
+  std::vector<int64_t> aoc(static_cast<size_t>(g), 0);
+  for (int i = 0; i < g; ++i) {
+    const int rep = q[static_cast<size_t>(i)];
+    for (const auto& [t, e] : view[static_cast<size_t>(i)]) {
+      aoc[static_cast<size_t>(i)] +=
+          EncodeCost(rep, size[static_cast<size_t>(i)], t, sizes_by_rep_[static_cast<size_t>(t)], e);
+      buckets[t].push_back({i, e});
+    }
+  }
+  std::unordered_map<uint64_t, int64_t> proxy;
+  for (const auto& [t, entries] : buckets) {  // :1863-1903
+    if (entries.size() < 2) continue;
+    const int64_t st = sizes_by_rep_[static_cast<size_t>(t)];
+    for (size_t x = 0; x < entries.size(); ++x) {
+      const int ai = entries[x].i;
+      const int ra = q[static_cast<size_t>(ai)];
+      if (t == ra) continue;
+      for (size_t y = x + 1; y < entries.size(); ++y) {
+        const int bi = entries[y].i;
+        const int rb = q[static_cast<size_t>(bi)];
+        if (t == rb) continue;
+        if (proxy_examined) ++*proxy_examined;
+        const int64_t sa = size[static_cast<size_t>(ai)], sb = size[static_cast<size_t>(bi)];
+        const int64_t old_cost = EncodeCost(ra, sa, t, st, entries[x].e) + EncodeCost(rb, sb, t, st, entries[y].e);
+        const int64_t new_cost = EncodeCost(ra, sa + sb, t, st, entries[x].e + entries[y].e);
+        const int64_t delta = old_cost - new_cost;
+        if (delta <= 0) continue;
+        proxy[PairKey(ai, bi)] += delta;
+      }
+    }
+  }
+  for (int i = 0; i < g; ++i) {  // :1905-1924 direct edges
+    for (const auto& [t, e] : view[static_cast<size_t>(i)]) {
+      const auto it = index_by_rep.find(t);
+      if (it == index_by_rep.end()) continue;
+      const int j = it->second;
+      if (i >= j) continue;
+      proxy[PairKey(i, j)] += e;
+    }
+  }
+  struct Ranked {
+    int other;
+    double ratio;
+    int64_t gain;
+    int lo, hi;
+  };
+  std::vector<std::vector<Ranked>> ranked(static_cast<size_t>(g));
+  for (const auto& [key, gain] : proxy) {  // :1942-1962
+    if (gain <= 0) continue;
+    const int a = static_cast<int>(key >> 32U), b = static_cast<int>(key & 0xffffffffU);
+    const int ra = q[static_cast<size_t>(a)], rb = q[static_cast<size_t>(b)];
+    const double ratio = static_cast<double>(gain) /
+                         static_cast<double>(std::max<int64_t>(1, aoc[static_cast<size_t>(a)] + aoc[static_cast<size_t>(b)]));
+    ranked[static_cast<size_t>(a)].push_back({b, ratio, gain, std::min(ra, rb), std::max(ra, rb)});
+    ranked[static_cast<size_t>(b)].push_back({a, ratio, gain, std::min(ra, rb), std::max(ra, rb)});
+  }
+  std::vector<uint64_t> keys;  // :1964-1989
+  for (int i = 0; i < g; ++i) {
+    auto& r = ranked[static_cast<size_t>(i)];
+    std::sort(r.begin(), r.end(), [](const Ranked& l, const Ranked& h) {  // :1926-1938
+      if (l.ratio != h.ratio) return l.ratio > h.ratio;
+      if (l.gain != h.gain) return l.gain > h.gain;
+      if (l.lo != h.lo) return l.lo < h.lo;
+      return l.hi < h.hi;
+    });
+    const size_t keep = std::min(r.size(), static_cast<size_t>(top_k));
+    for (size_t p = 0; p < keep; ++p) keys.push_back(PairKey(i, r[p].other));
+  }
+  std::sort(keys.begin(), keys.end());
+  keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+  for (uint64_t k : keys) out.push_back({static_cast<int>(k >> 32U), static_cast<int>(k & 0xffffffffU), 0, 0});
+  return out;
+}
+
+// Scoring (scoring_cpu.cpp:67-116), threshold filter (sweg.cpp:2146-2186) and
+// greedy endpoint-disjoint selection (sweg.cpp:2188-2281) for one group.
+void Sweg::ScoreFilterSelect(const std::vector<int>& q, std::vector<CandidatePair>* pairs, double theta,
+                             std::vector<std::pair<int, int>>* selected) {
+  if (pairs->empty()) return;
+  stats_.merge_exact_gain_calls += pairs->size();
+  cur_.exact_gain_calls += pairs->size();
+  for (auto& p : *pairs) {
+    const GainResult r = q_.Gain(q[static_cast<size_t>(p.a_idx)], q[static_cast<size_t>(p.b_idx)]);
+    p.gain = r.gain;
+    p.before = r.before;
+  }
+  std::vector<CandidatePair> passed;
+  for (const auto& p : *pairs) {
+    const double ratio = p.before > 0 ? static_cast<double>(p.gain) / static_cast<double>(p.before) : 0.0;
+    if (p.gain <= 0) {
+      stats_.merge_rejected_by_threshold += 1;
+      continue;
+    }
+    cur_.exact_gain_positive_count += 1;
+    if (opt_.ea_use_threshold) {
+      if (p.before <= 0 || ratio < theta) {
+        stats_.merge_rejected_by_threshold += 1;
+        continue;
+      }
+    }
+    stats_.merge_positive_gain_pairs += 1;
+    cur_.above_threshold_count += 1;
+    passed.push_back(p);
+  }
+  *pairs = passed;
+  if (passed.empty()) return;
+  std::vector<CandidatePair> order = passed;
+  std::sort(order.begin(), order.end(), [](const CandidatePair& l, const CandidatePair& r) {
+    if (l.gain != r.gain) return l.gain > r.gain;
+    if (l.a_idx != r.a_idx) return l.a_idx < r.a_idx;
+    return l.b_idx < r.b_idx;
+  });
+  std::vector<char> used(q.size(), 0);
+  int64_t total = 0;
+  size_t chosen = 0;
+  for (const auto& p : order) {
+    if (used[static_cast<size_t>(p.a_idx)] || used[static_cast<size_t>(p.b_idx)]) {
+      stats_.merge_rejected_by_overlap += 1;
+      continue;
+    }
+    used[static_cast<size_t>(p.a_idx)] = used[static_cast<size_t>(p.b_idx)] = 1;
+    selected->emplace_back(q[static_cast<size_t>(p.a_idx)], q[static_cast<size_t>(p.b_idx)]);
+    total += p.gain;
+    stats_.isolated_gain_sum += p.gain;
+    ++chosen;
+  }
+  stats_.merge_candidate_pairs += static_cast<int64_t>(passed.size());
+  stats_.merge_selected_pairs += static_cast<int64_t>(chosen);
+  stats_.merge_total_local_gain += total;
+  cur_.matching_selected_count += chosen;
+}
+
+void Sweg::UpdateS(int a, int b) {  // sweg.cpp:2977-3015
+  if (I_[static_cast<size_t>(a)] == -1 || I_[static_cast<size_t>(b)] == -1) return;
+  q_.Merge(a, b);
+  cur_.actual_merge_count += 1;
+  int last = a, count_a = 0, count_b = 0;
+  for (int v = I_[static_cast<size_t>(a)]; v != -1; v = J_[static_cast<size_t>(v)]) {
+    last = v;
+    ++count_a;
+  }
+  for (int v = I_[static_cast<size_t>(b)]; v != -1; v = J_[static_cast<size_t>(v)]) {
+    S_[static_cast<size_t>(v)] = a;
+    ++count_b;
+  }
+  J_[static_cast<size_t>(last)] = I_[static_cast<size_t>(b)];
+  I_[static_cast<size_t>(b)] = -1;
+  sizes_by_rep_[static_cast<size_t>(a)] = count_a + count_b;
+  sizes_by_rep_[static_cast<size_t>(b)] = 0;
+}
+
+// sweg.cpp:2665-2769 (g0 policy): every selected pair is re-scored on the
+// block-start state (the loop at :2683-2719 runs before any Update_S), the
+// positive ones are kept, and only then are they merged in list order
+// (:2745-2747).
+void Sweg::Commit(const std::vector<std::pair<int, int>>& pairs) {
+  std::vector<std::pair<int, int>> accepted;
+  for (const auto& [a, b] : pairs) {
+    ++stats_.selected_merges_for_validation;
+    if (I_[static_cast<size_t>(a)] == -1 || I_[static_cast<size_t>(b)] == -1) continue;
+    uint64_t raw = 0;
+    const GainResult r = q_.Gain(a, b, &raw);
+    ++stats_.validation_exact_calls;
+    stats_.validation_exact_row_entry_work += raw;
+    if (r.gain > 0) {
+      accepted.emplace_back(a, b);
+    } else {
+      ++stats_.rejected_nonpositive;
+    }
+  }
+  for (const auto& [a, b] : accepted) UpdateS(a, b);
+  stats_.accepted_merges_after_validation += accepted.size();
+  if (record_commits_) commit_log_.push_back(accepted);
+}
+
+void Sweg::Merge(int iter) {  // sweg.cpp:953-1145
+  const double theta = 1.0 / static_cast<double>(iter + 1);  // reciprocal policy, :400-406
+  const std::vector<std::pair<int, int>> groups = groups_;
+  const size_t block = static_cast<size_t>(opt_.group_batch_size);
+  for (size_t b0 = 0; b0 < groups.size(); b0 += block) {
+    const size_t b1 = std::min(groups.size(), b0 + block);
+    std::vector<std::vector<int>> items;
+    for (size_t gi = b0; gi < b1; ++gi) {
+      if (groups[gi].second < 2) continue;
+      std::vector<int> q(active_reps_.begin() + groups[gi].first,
+                         active_reps_.begin() + groups[gi].first + groups[gi].second);
+      items.push_back(std::move(q));
+    }
+    std::vector<std::vector<CandidatePair>> prepared(items.size());
+    for (size_t i = 0; i < items.size(); ++i) {
+      uint64_t examined = 0;
+      prepared[i] = PrepareGroup(items[i], &examined);
+      const uint64_t g = items[i].size();
+      stats_.merge_group_count += 1;
+      stats_.merge_group_max_size = std::max<uint64_t>(stats_.merge_group_max_size, g);
+      stats_.merge_raw_pair_count += g * (g - 1) / 2;
+      stats_.merge_candidate_pairs_after_prune += prepared[i].size();
+      cur_.candidate_proxy_pairs_examined += examined;
+      cur_.candidate_pairs_after_topk += prepared[i].size();
+    }
+    std::vector<std::pair<int, int>> selected;
+    for (size_t i = 0; i < items.size(); ++i) {
+      std::vector<std::pair<int, int>> group_sel;
+      ScoreFilterSelect(items[i], &prepared[i], theta, &group_sel);
+      selected.insert(selected.end(), group_sel.begin(), group_sel.end());
+    }
+    Commit(selected);
+  }
+}
+
+void Sweg::Run(int iterations) {  // sweg.cpp:651-709
+  for (int iter = 1; iter <= iterations; ++iter) {
+    Divide(iter);
+    cur_ = IterationStats{};
+    cur_.iteration = iter;
+    cur_.active_supernodes = static_cast<int64_t>(active_reps_.size());
+    cur_.group_count = static_cast<int64_t>(groups_.size());
+    Merge(iter);
+    rounds_.push_back(cur_);
+  }
+}
+
+std::vector<int> Sweg::Labels() const {  // sweg.cpp:2902-2918
+  std::vector<int> labels(static_cast<size_t>(n_), -1);
+  for (int rep = 0; rep < n_; ++rep) {
+    for (int v = I_[static_cast<size_t>(rep)]; v != -1; v = J_[static_cast<size_t>(v)]) labels[static_cast<size_t>(v)] = rep;
+  }
+  if (std::find(labels.begin(), labels.end(), -1) != labels.end()) {
+    throw std::runtime_error("Partition contains an unlabeled node");
+  }
+  return labels;
+}
+
+uint64_t Sweg::PartitionHash() const {  // sweg.cpp:2920-2928 (FNV-1a 64)
+  uint64_t h = 1469598103934665603ULL;
+  for (int label : Labels()) {
+    h ^= static_cast<uint64_t>(static_cast<uint32_t>(label));
+    h *= 1099511628211ULL;
+  }
+  return h;
+}
+
+int64_t Sweg::ExactCurrentPartitionCost() const { return ExactPartitionCost(*g_, Labels()); }
+
+// sweg.cpp:3017-3163 with the mags-compatible objective: relabel supernodes by
+// ascending rep, count each block's edges (internal edges once, u < v), keep
+// the block sparse (Cp) when e <= 1 + cap - e, else emit P plus the missing
+// pairs as Cm; P, Cp and Cm are sorted.
+Encoding Sweg::Encode() const {
+  Encoding r;
+  r.node_to_supernode.assign(static_cast<size_t>(n_), -1);
+  for (int rep = 0; rep < n_; ++rep) {
+    if (I_[static_cast<size_t>(rep)] == -1) continue;
+    std::vector<int> nodes;
+    for (int v = I_[static_cast<size_t>(rep)]; v != -1; v = J_[static_cast<size_t>(v)]) nodes.push_back(v);
+    const int id = static_cast<int>(r.supernodes.size());
+    r.supernode_sizes.push_back(static_cast<int>(nodes.size()));
+    for (int v : nodes) r.node_to_supernode[static_cast<size_t>(v)] = id;
+    r.supernodes.push_back(std::move(nodes));
+  }
+  struct Arc {
+    uint64_t key;
+    int u, v;
+  };
+  std::vector<Arc> arcs;
+  for (int u = 0; u < n_; ++u) {
+    const int su = r.node_to_supernode[static_cast<size_t>(u)];
+    for (int64_t k = g_->row_ptr[static_cast<size_t>(u)]; k < g_->row_ptr[static_cast<size_t>(u) + 1]; ++k) {
+      const int v = g_->col_idx[static_cast<size_t>(k)];
+      const int sv = r.node_to_supernode[static_cast<size_t>(v)];
+      if (su > sv) continue;
+      if (su == sv && u >= v) continue;
+      arcs.push_back({(static_cast<uint64_t>(static_cast<uint32_t>(su)) << 32U) | static_cast<uint32_t>(sv), u, v});
+    }
+  }
+  std::sort(arcs.begin(), arcs.end(), [](const Arc& a, const Arc& b) {
+    if (a.key != b.key) return a.key < b.key;
+    if (a.u != b.u) return a.u < b.u;
+    return a.v < b.v;
+  });
+  size_t begin = 0;
+  while (begin < arcs.size()) {
+    size_t end = begin + 1;
+    while (end < arcs.size() && arcs[end].key == arcs[begin].key) ++end;
+    const int A = static_cast<int>(arcs[begin].key >> 32U), B = static_cast<int>(arcs[begin].key & 0xffffffffU);
+    const int64_t e = static_cast<int64_t>(end - begin);
+    const int64_t sa = r.supernode_sizes[static_cast<size_t>(A)], sb = r.supernode_sizes[static_cast<size_t>(B)];
+    const int64_t cap = A == B ? sa * (sa - 1) / 2 : sa * sb;
+    if (e <= 1 + cap - e) {
+      for (size_t i = begin; i < end; ++i) r.Cp.emplace_back(arcs[i].u, arcs[i].v);
+    } else {
+      r.P.emplace_back(A, B);
+      std::unordered_set<uint64_t> present;
+      for (size_t i = begin; i < end; ++i) {
+        present.insert((static_cast<uint64_t>(static_cast<uint32_t>(arcs[i].u)) << 32U) |
+                       static_cast<uint32_t>(arcs[i].v));
+      }
+      for (int u : r.supernodes[static_cast<size_t>(A)]) {
+        for (int v : r.supernodes[static_cast<size_t>(B)]) {
+          if (A == B && u >= v) continue;
+          const uint64_t k = (static_cast<uint64_t>(static_cast<uint32_t>(u)) << 32U) | static_cast<uint32_t>(v);
+          if (!present.count(k)) r.Cm.emplace_back(u, v);
+        }
+      }
+    }
+    begin = end;
+  }
+  std::sort(r.P.begin(), r.P.end());
+  std::sort(r.Cp.begin(), r.Cp.end());
+  std::sort(r.Cm.begin(), r.Cm.end());
+  return r;
+}
+
+}  // namespace oracle

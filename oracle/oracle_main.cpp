@@ -1,0 +1,77 @@
+// Command-line driver for the CPU oracle -- TEST INFRASTRUCTURE.  Prints the
+// reference's `metric.<key>=<value>` lines (main.cpp:271-273, 577-956) for the
+// keys the parity checks compare.
+//   beam_oracle <edge-list> <seed> [iterations]
+#include <cstdio>
+#include <cstdlib>
+#include <exception>
+
+#include "oracle.hpp"
+
+int main(int argc, char** argv) {
+  if (argc < 3) {
+    std::fprintf(stderr, "usage: beam_oracle <edge-list> <seed> [iterations]\n");
+    return 2;
+  }
+  try {
+    const oracle::Graph g = oracle::LoadEdgeList(argv[1]);
+    oracle::Options opt;
+    opt.seed = std::strtoull(argv[2], nullptr, 10);
+    const int iterations = argc > 3 ? std::atoi(argv[3]) : 20;
+    oracle::Sweg sweg(g, opt);
+    sweg.Run(iterations);
+    const oracle::Encoding e = sweg.Encode();
+    const oracle::Stats& s = sweg.stats();
+    const long long cost = static_cast<long long>(e.P.size() + e.Cp.size() + e.Cm.size());
+    std::printf("metric.n=%d\n", g.n);
+    std::printf("metric.partition_hash=%llu\n", static_cast<unsigned long long>(sweg.PartitionHash()));
+    std::printf("metric.exact_partition_cost=%lld\n", static_cast<long long>(sweg.ExactCurrentPartitionCost()));
+    std::printf("metric.supernode_count=%zu\n", e.supernodes.size());
+    std::printf("metric.P=%zu\nmetric.Cp=%zu\nmetric.Cm=%zu\n", e.P.size(), e.Cp.size(), e.Cm.size());
+    std::printf("metric.encoding_cost_mags_x2=%lld\n", 2 * cost);
+    std::printf("metric.cost_ratio_mags_compatible=%.6f\n",
+                static_cast<double>(2 * cost) / (2.0 * static_cast<double>(g.input_edges_raw)));
+    std::printf("metric.divide_max_group_size=%lld\n", static_cast<long long>(s.divide_max_group_size));
+    std::printf("metric.divide_fallback_splits=%lld\n", static_cast<long long>(s.divide_fallback_splits));
+    std::printf("metric.merge_candidate_pairs=%lld\n", static_cast<long long>(s.merge_candidate_pairs));
+    std::printf("metric.merge_selected_pairs=%lld\n", static_cast<long long>(s.merge_selected_pairs));
+    std::printf("metric.merge_total_local_gain=%lld\n", static_cast<long long>(s.merge_total_local_gain));
+    std::printf("metric.merge_group_count=%llu\n", static_cast<unsigned long long>(s.merge_group_count));
+    std::printf("metric.merge_group_max_size=%llu\n", static_cast<unsigned long long>(s.merge_group_max_size));
+    std::printf("metric.merge_raw_pair_count=%llu\n", static_cast<unsigned long long>(s.merge_raw_pair_count));
+    std::printf("metric.merge_candidate_pairs_after_prune=%llu\n",
+                static_cast<unsigned long long>(s.merge_candidate_pairs_after_prune));
+    std::printf("metric.merge_exact_gain_calls=%llu\n", static_cast<unsigned long long>(s.merge_exact_gain_calls));
+    std::printf("metric.merge_positive_gain_pairs=%llu\n",
+                static_cast<unsigned long long>(s.merge_positive_gain_pairs));
+    std::printf("metric.merge_rejected_by_overlap=%llu\n",
+                static_cast<unsigned long long>(s.merge_rejected_by_overlap));
+    std::printf("metric.merge_rejected_by_threshold=%llu\n",
+                static_cast<unsigned long long>(s.merge_rejected_by_threshold));
+    std::printf("metric.isolated_gain_sum=%lld\n", static_cast<long long>(s.isolated_gain_sum));
+    std::printf("metric.selected_merges_for_validation=%llu\n",
+                static_cast<unsigned long long>(s.selected_merges_for_validation));
+    std::printf("metric.accepted_merges_after_validation=%llu\n",
+                static_cast<unsigned long long>(s.accepted_merges_after_validation));
+    std::printf("metric.rejected_nonpositive=%llu\n", static_cast<unsigned long long>(s.rejected_nonpositive));
+    std::printf("metric.validation_exact_calls=%llu\n", static_cast<unsigned long long>(s.validation_exact_calls));
+    std::printf("metric.validation_exact_row_entry_work=%llu\n",
+                static_cast<unsigned long long>(s.validation_exact_row_entry_work));
+    std::printf("metric.quotient_nnz_final=%llu\n", static_cast<unsigned long long>(sweg.Q().Nnz()));
+    for (const auto& r : sweg.rounds()) {
+      std::printf("round %d %lld %lld %llu %llu %llu %llu %llu %llu %llu\n", r.iteration,
+                  static_cast<long long>(r.active_supernodes), static_cast<long long>(r.group_count),
+                  static_cast<unsigned long long>(r.candidate_proxy_pairs_examined),
+                  static_cast<unsigned long long>(r.candidate_pairs_after_topk),
+                  static_cast<unsigned long long>(r.exact_gain_calls),
+                  static_cast<unsigned long long>(r.exact_gain_positive_count),
+                  static_cast<unsigned long long>(r.above_threshold_count),
+                  static_cast<unsigned long long>(r.matching_selected_count),
+                  static_cast<unsigned long long>(r.actual_merge_count));
+    }
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "Error: %s\n", e.what());
+    return 1;
+  }
+  return 0;
+}
