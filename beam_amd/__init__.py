@@ -1,0 +1,317 @@
+"""Python plumbing over the HIP merge-path C-ABI (include/beam_hip.h).
+
+The product is libbeam_hip.so, built in-tree by ``make -C beam_amd``.  This
+module only loads it and moves NumPy buffers across the C boundary; it never
+computes any part of the path itself and raises when the library is missing.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libbeam_hip.so")
+
+STAT_NAMES = [
+    "divide_max_group_size",
+    "divide_fallback_splits",
+    "merge_candidate_pairs",
+    "merge_selected_pairs",
+    "merge_total_local_gain",
+    "merge_group_count",
+    "merge_group_max_size",
+    "merge_raw_pair_count",
+    "merge_candidate_pairs_after_prune",
+    "merge_exact_gain_calls",
+    "merge_positive_gain_pairs",
+    "merge_rejected_by_overlap",
+    "merge_rejected_by_threshold",
+    "selected_merges_for_validation",
+    "accepted_merges_after_validation",
+    "rejected_nonpositive",
+    "validation_exact_calls",
+    "validation_exact_row_entry_work",
+    "isolated_gain_sum",
+    "blocks",
+]
+
+ROUND_NAMES = [
+    "iteration",
+    "active_supernodes",
+    "group_count",
+    "candidate_proxy_pairs_examined",
+    "candidate_pairs_after_topk",
+    "exact_gain_calls",
+    "exact_gain_positive_count",
+    "above_threshold_count",
+    "matching_selected_count",
+    "actual_merge_count",
+]
+
+MAX_SELECT = 256
+
+
+class BeamHipError(RuntimeError):
+    pass
+
+
+class Options(ctypes.Structure):
+    _fields_ = [
+        ("top_k", ctypes.c_int32),
+        ("group_batch_size", ctypes.c_int32),
+        ("divide_hash_dims", ctypes.c_int32),
+        ("divide_max_group", ctypes.c_int32),
+        ("ea_use_threshold", ctypes.c_int32),
+        ("device", ctypes.c_int32),
+        ("seed", ctypes.c_uint64),
+    ]
+
+
+_lib = None
+
+
+def library() -> ctypes.CDLL:
+    """Loads libbeam_hip.so; raises BeamHipError when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise BeamHipError(f"{LIB_PATH} is missing: build it with `make -C beam_amd`")
+    lib = ctypes.CDLL(LIB_PATH)
+    P = ctypes.c_void_p
+    i32, i64, f64 = ctypes.c_int32, ctypes.c_int64, ctypes.c_double
+    sigs = {
+        "beam_hip_build_mode": (ctypes.c_char_p, []),
+        "beam_hip_compiled": (ctypes.c_int, []),
+        "beam_hip_last_error": (ctypes.c_char_p, []),
+        "beam_hip_create": (ctypes.c_int, [P, P, i32, i64, ctypes.POINTER(Options), ctypes.POINTER(P)]),
+        "beam_hip_destroy": (None, [P]),
+        "beam_hip_run": (ctypes.c_int, [P, i32]),
+        "beam_hip_iterate": (ctypes.c_int, [P, i32]),
+        "beam_hip_labels": (ctypes.c_int, [P, P]),
+        "beam_hip_stats": (ctypes.c_int, [P, P]),
+        "beam_hip_round_stats": (ctypes.c_int, [P, P, i32]),
+        "beam_hip_timing": (ctypes.c_int, [P, P]),
+        "beam_hip_encoding_counts": (ctypes.c_int, [P, P]),
+        "beam_hip_upload_state": (ctypes.c_int, [P, P, P, P, P, P, P, P]),
+        "beam_hip_nnz": (i64, [P]),
+        "beam_hip_download_state": (ctypes.c_int, [P, P, P, P, P, P, P]),
+        "beam_hip_score_pairs": (ctypes.c_int, [P, P, P, i64, P, P]),
+        "beam_hip_merge_groups": (
+            ctypes.c_int,
+            [P, P, i32, P, P, i32, f64, i32, P, P, P, i32, P, P],
+        ),
+        "beam_hip_divide": (ctypes.c_int, [P, i32, P, P, P, P]),
+        "beam_hip_commit": (ctypes.c_int, [P, P, P, i32]),
+        "beam_hip_reset": (ctypes.c_int, [P]),
+        "beam_hip_profile": (ctypes.c_int, [P, i32]),
+        "beam_hip_kernel_profile": (ctypes.c_int, [P, P]),
+        "beam_hip_partition_hash": (ctypes.c_uint64, [P, i32]),
+        "beam_graph_load": (P, [ctypes.c_char_p]),
+        "beam_graph_last_error": (ctypes.c_char_p, []),
+        "beam_graph_info": (None, [P, P, P, P]),
+        "beam_graph_csr": (None, [P, P, P]),
+        "beam_graph_free": (None, [P]),
+    }
+    for name, (res, args) in sigs.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def exported_symbols() -> list[str]:
+    """Names of the C entry points this wrapper binds."""
+    return [
+        "beam_hip_build_mode", "beam_hip_compiled", "beam_hip_last_error", "beam_hip_create",
+        "beam_hip_destroy", "beam_hip_run", "beam_hip_iterate", "beam_hip_labels", "beam_hip_stats",
+        "beam_hip_round_stats", "beam_hip_timing", "beam_hip_encoding_counts", "beam_hip_upload_state",
+        "beam_hip_nnz", "beam_hip_download_state", "beam_hip_score_pairs", "beam_hip_merge_groups",
+        "beam_hip_divide", "beam_hip_commit", "beam_hip_reset", "beam_hip_profile",
+        "beam_hip_kernel_profile", "beam_hip_partition_hash", "beam_graph_load", "beam_graph_last_error",
+        "beam_graph_info", "beam_graph_csr", "beam_graph_free",
+    ]
+
+
+def _ptr(a: np.ndarray) -> ctypes.c_void_p:
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _check(rc: int) -> int:
+    if rc < 0:
+        raise BeamHipError(library().beam_hip_last_error().decode())
+    return rc
+
+
+@dataclass
+class Graph:
+    n: int
+    arcs: int
+    input_edges_raw: int
+    row_ptr: np.ndarray  # int64 [n + 1]
+    col: np.ndarray  # int32 [arcs]
+
+
+def load_graph(path: str) -> Graph:
+    """Reads an edge list with the reference loader's semantics (C++ loader in the library)."""
+    lib = library()
+    h = lib.beam_graph_load(path.encode())
+    if not h:
+        raise BeamHipError(lib.beam_graph_last_error().decode())
+    try:
+        n = ctypes.c_int()
+        arcs = ctypes.c_longlong()
+        raw = ctypes.c_longlong()
+        lib.beam_graph_info(h, ctypes.byref(n), ctypes.byref(arcs), ctypes.byref(raw))
+        row_ptr = np.empty(n.value + 1, dtype=np.int64)
+        col = np.empty(max(arcs.value, 1), dtype=np.int32)
+        lib.beam_graph_csr(h, _ptr(row_ptr), _ptr(col))
+    finally:
+        lib.beam_graph_free(h)
+    return Graph(n.value, arcs.value, raw.value, row_ptr, col[: arcs.value])
+
+
+class Engine:
+    """Device-resident merge engine over one graph (one GPU)."""
+
+    def __init__(self, graph: Graph, seed: int, top_k: int = 16, group_batch_size: int = 64,
+                 divide_hash_dims: int = 16, divide_max_group: int = 512, device: int = 0):
+        self.lib = library()
+        self.n = graph.n
+        opt = Options(top_k, group_batch_size, divide_hash_dims, divide_max_group, 1, device, seed)
+        h = ctypes.c_void_p()
+        row_ptr = np.ascontiguousarray(graph.row_ptr, dtype=np.int64)
+        col = np.ascontiguousarray(graph.col, dtype=np.int32)
+        _check(self.lib.beam_hip_create(_ptr(row_ptr), _ptr(col), graph.n, graph.arcs,
+                                        ctypes.byref(opt), ctypes.byref(h)))
+        self.h = h
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.beam_hip_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def run(self, iterations: int) -> None:
+        _check(self.lib.beam_hip_run(self.h, iterations))
+
+    def reset(self) -> None:
+        _check(self.lib.beam_hip_reset(self.h))
+
+    def set_profile(self, enable: bool) -> None:
+        _check(self.lib.beam_hip_profile(self.h, 1 if enable else 0))
+
+    def kernel_profile(self) -> dict:
+        out = np.zeros(5, dtype=np.float64)
+        _check(self.lib.beam_hip_kernel_profile(self.h, _ptr(out)))
+        return {"group_kernel_ms": float(out[0]), "group_kernel_launches": int(out[1]),
+                "member_rows": int(out[2]), "member_entries": int(out[3]), "pairs_scored": int(out[4])}
+
+    def iterate(self, it: int) -> None:
+        _check(self.lib.beam_hip_iterate(self.h, it))
+
+    def labels(self) -> np.ndarray:
+        out = np.empty(self.n, dtype=np.int32)
+        _check(self.lib.beam_hip_labels(self.h, _ptr(out)))
+        return out
+
+    def stats(self) -> dict:
+        out = np.zeros(len(STAT_NAMES), dtype=np.int64)
+        _check(self.lib.beam_hip_stats(self.h, _ptr(out)))
+        return dict(zip(STAT_NAMES, (int(v) for v in out)))
+
+    def rounds(self, max_rounds: int = 4096) -> list[dict]:
+        out = np.zeros(10 * max_rounds, dtype=np.int64)
+        k = _check(self.lib.beam_hip_round_stats(self.h, _ptr(out), max_rounds))
+        return [dict(zip(ROUND_NAMES, (int(v) for v in out[10 * i: 10 * i + 10]))) for i in range(k)]
+
+    def timing(self) -> dict:
+        out = np.zeros(3, dtype=np.float64)
+        _check(self.lib.beam_hip_timing(self.h, _ptr(out)))
+        return {"run_ms": float(out[0]), "divide_ms": float(out[1]), "merge_ms": float(out[2])}
+
+    def encoding_counts(self) -> dict:
+        out = np.zeros(6, dtype=np.int64)
+        _check(self.lib.beam_hip_encoding_counts(self.h, _ptr(out)))
+        keys = ["P", "Cp", "Cm", "P_loops", "supernode_count", "exact_partition_cost"]
+        return dict(zip(keys, (int(v) for v in out)))
+
+    def upload_state(self, offsets, nbr, cnt, sizes, internal, active, labels) -> None:
+        arrs = [np.ascontiguousarray(offsets, np.int64), np.ascontiguousarray(nbr, np.int32),
+                np.ascontiguousarray(cnt, np.int32), np.ascontiguousarray(sizes, np.int32),
+                np.ascontiguousarray(internal, np.int32), np.ascontiguousarray(active, np.uint8),
+                np.ascontiguousarray(labels, np.int32)]
+        _check(self.lib.beam_hip_upload_state(self.h, *[_ptr(a) for a in arrs]))
+
+    def download_state(self):
+        nnz = int(self.lib.beam_hip_nnz(self.h))
+        if nnz < 0:
+            raise BeamHipError(self.lib.beam_hip_last_error().decode())
+        offsets = np.zeros(self.n + 1, np.int64)
+        nbr = np.zeros(max(nnz, 1), np.int32)
+        cnt = np.zeros(max(nnz, 1), np.int32)
+        sizes = np.zeros(self.n, np.int32)
+        internal = np.zeros(self.n, np.int32)
+        active = np.zeros(self.n, np.uint8)
+        _check(self.lib.beam_hip_download_state(self.h, _ptr(offsets), _ptr(nbr), _ptr(cnt), _ptr(sizes),
+                                                _ptr(internal), _ptr(active)))
+        return offsets, nbr[:nnz], cnt[:nnz], sizes, internal, active
+
+    def score_pairs(self, a, b):
+        a = np.ascontiguousarray(a, np.int32)
+        b = np.ascontiguousarray(b, np.int32)
+        gain = np.zeros(len(a), np.int64)
+        before = np.zeros(len(a), np.int64)
+        _check(self.lib.beam_hip_score_pairs(self.h, _ptr(a), _ptr(b), len(a), _ptr(gain), _ptr(before)))
+        return gain, before
+
+    def merge_groups(self, reps, starts, lengths, theta: float, commit: bool, cand_cap: int = 8192):
+        reps = np.ascontiguousarray(reps, np.int32)
+        starts = np.ascontiguousarray(starts, np.int32)
+        lengths = np.ascontiguousarray(lengths, np.int32)
+        g = len(starts)
+        pairs = np.zeros(2 * g * cand_cap, np.int32)
+        gains = np.zeros(2 * g * cand_cap, np.int64)
+        counts = np.zeros(g, np.int32)
+        sel = np.zeros(2 * g * MAX_SELECT, np.int32)
+        sel_count = np.zeros(g, np.int32)
+        _check(self.lib.beam_hip_merge_groups(self.h, _ptr(reps), len(reps), _ptr(starts), _ptr(lengths), g,
+                                              float(theta), 1 if commit else 0, _ptr(pairs), _ptr(gains),
+                                              _ptr(counts), cand_cap, _ptr(sel), _ptr(sel_count)))
+        out = []
+        for i in range(g):
+            c = int(counts[i])
+            base = i * cand_cap
+            p = pairs[2 * base: 2 * (base + c)].reshape(-1, 2)
+            gb = gains[2 * base: 2 * (base + c)].reshape(-1, 2)
+            s = sel[2 * i * MAX_SELECT: 2 * (i * MAX_SELECT + int(sel_count[i]))].reshape(-1, 2)
+            out.append({"pairs": p.copy(), "gain": gb[:, 0].copy(), "before": gb[:, 1].copy(),
+                        "selected": s.copy()})
+        return out
+
+    def divide(self, it: int):
+        order = np.zeros(self.n, np.int32)
+        starts = np.zeros(self.n + 1, np.int32)
+        lengths = np.zeros(self.n + 1, np.int32)
+        ng = ctypes.c_int32()
+        _check(self.lib.beam_hip_divide(self.h, it, _ptr(order), _ptr(starts), _ptr(lengths), ctypes.byref(ng)))
+        g = ng.value
+        total = int(lengths[:g].sum())
+        return order[:total], starts[:g].copy(), lengths[:g].copy()
+
+    def commit(self, keeps, removes) -> None:
+        keeps = np.ascontiguousarray(keeps, np.int32)
+        removes = np.ascontiguousarray(removes, np.int32)
+        _check(self.lib.beam_hip_commit(self.h, _ptr(keeps), _ptr(removes), len(keeps)))
+
+
+def partition_hash(labels: np.ndarray) -> int:
+    """FNV-1a 64 over node labels (Sweg::PartitionHash, src/sweg.cpp:2920-2928)."""
+    labels = np.ascontiguousarray(labels, np.int32)
+    return int(library().beam_hip_partition_hash(_ptr(labels), len(labels)))

@@ -1,0 +1,156 @@
+// Shared device helpers for the BEAM merge path on gfx950 (CDNA4, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace beam {
+
+constexpr int kWave = 64;
+
+// Device error bits, reported through beam_hip_last_error().
+enum : int {
+  kErrCapacity = 1,        // block edge count exceeds its capacity (cost_oracle.cpp:106-108)
+  kErrAsymmetric = 2,      // a-b count differs between the two rows (quotient_graph.cpp:427-429)
+  kErrScratch = 4,         // per-block scratch pool exhausted
+  kErrPool = 8,            // quotient entry pool exhausted
+  kErrSpecials = 16,       // too many merge endpoints in one neighbour row
+  kErrSelection = 32,      // group selection overflowed its slot
+};
+
+// Quotient row entry: neighbour supernode and edge count.
+struct Entry {
+  int32_t nbr;
+  int32_t cnt;
+};
+
+// min(e, 1 + cap - e) with cap = s(s-1)/2 for an internal block and sa*sb
+// otherwise (cost_oracle.cpp:93-110).  Sizes are < 2^31, so every capacity
+// fits in int64 without the reference's __int128 widening; an edge count above
+// the capacity raises kErrCapacity instead of throwing.
+__device__ __forceinline__ int64_t BlockCost(int64_t sa, int64_t sb, int64_t e, bool internal, int* err) {
+  const int64_t cap = internal ? (sa * (sa - 1)) / 2 : sa * sb;
+  if (e < 0 || e > cap) atomicOr(err, kErrCapacity);
+  const int64_t comp = 1 + cap - e;
+  return e < comp ? e : comp;
+}
+
+// Unchecked proxy cost min(e, 1 + cap - e) (sweg.cpp:1423-1433); negative for
+// a dense self entry, exactly as the reference computes it.
+__device__ __forceinline__ int64_t EncodeCost(int rep_u, int64_t su, int rep_x, int64_t sx, int64_t e) {
+  const int64_t cap = rep_u == rep_x ? (su * (su - 1)) / 2 : su * sx;
+  const int64_t comp = 1 + cap - e;
+  return e < comp ? e : comp;
+}
+
+__device__ __forceinline__ uint64_t SplitMix64(uint64_t x) {  // sweg.cpp:2290-2295
+  x += 0x9e3779b97f4a7c15ULL;
+  x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  x = (x ^ (x >> 27)) * 0x94d049bb133111ebULL;
+  return x ^ (x >> 31);
+}
+
+__device__ __forceinline__ uint64_t HashRank64(uint64_t seed, int node, int dim, int iter) {  // sweg.cpp:731-737
+  uint64_t x = seed;
+  x ^= 0x9e3779b97f4a7c15ULL * static_cast<uint64_t>(node + 1);
+  x ^= 0xbf58476d1ce4e5b9ULL * static_cast<uint64_t>(dim + 1);
+  x ^= 0x94d049bb133111ebULL * static_cast<uint64_t>(iter + 1);
+  return SplitMix64(x);
+}
+
+__device__ __forceinline__ uint64_t FallbackRank(uint64_t seed, int rep, int iter) {  // sweg.cpp:746-752
+  uint64_t x = seed;
+  x ^= 0x9e3779b97f4a7c15ULL * static_cast<uint64_t>(rep + 1);
+  x ^= 0xbf58476d1ce4e5b9ULL * static_cast<uint64_t>(iter + 1);
+  x ^= 0xd6e8feb86659fd93ULL;
+  return SplitMix64(x);
+}
+
+__device__ __forceinline__ int LaneId() { return static_cast<int>(threadIdx.x & (kWave - 1)); }
+__device__ __forceinline__ int WaveId() { return static_cast<int>(threadIdx.x / kWave); }
+
+__device__ __forceinline__ int64_t WaveSum64(int64_t v) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+__device__ __forceinline__ int WaveSum32(int v) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+__device__ __forceinline__ int WaveMax32(int v) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, kWave));
+  return v;
+}
+
+// Inclusive scan across a wave.
+__device__ __forceinline__ uint32_t WaveInclusiveScan(uint32_t v) {
+  const int lane = LaneId();
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const uint32_t t = __shfl_up(v, off, kWave);
+    if (lane >= off) v += t;
+  }
+  return v;
+}
+
+// Exclusive scan of one value per thread over the whole workgroup; `wave_tot`
+// is LDS scratch of blockDim/64 + 1 words.  Returns the exclusive prefix and
+// writes the workgroup total to *total.  Every thread must call it.
+__device__ __forceinline__ uint32_t BlockExclusiveScan(uint32_t v, uint32_t* wave_tot, uint32_t* total) {
+  const int lane = LaneId(), wave = WaveId();
+  const int nwaves = static_cast<int>(blockDim.x) / kWave;
+  const uint32_t inc = WaveInclusiveScan(v);
+  if (lane == kWave - 1) wave_tot[wave] = inc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t run = 0;
+    for (int w = 0; w < nwaves; ++w) {
+      const uint32_t t = wave_tot[w];
+      wave_tot[w] = run;
+      run += t;
+    }
+    wave_tot[nwaves] = run;
+  }
+  __syncthreads();
+  const uint32_t res = wave_tot[wave] + inc - v;
+  *total = wave_tot[nwaves];
+  __syncthreads();
+  return res;
+}
+
+__device__ __forceinline__ uint32_t HashInt(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352dU;
+  x ^= x >> 15;
+  x *= 0x846ca68bU;
+  x ^= x >> 16;
+  return x;
+}
+
+// Device counters accumulated per iteration (RuntimeStats / IterationProfile
+// fields, sweg.hpp:83-386).
+struct Counters {
+  unsigned long long group_count;              // groups with >= 2 members
+  unsigned long long group_max_size;
+  unsigned long long raw_pair_count;
+  unsigned long long candidate_pairs_after_prune;
+  unsigned long long proxy_pairs_examined;
+  unsigned long long exact_gain_calls;
+  unsigned long long positive_gain_count;      // gain > 0
+  unsigned long long above_threshold;          // passed the filter
+  unsigned long long rejected_by_threshold;
+  unsigned long long rejected_by_overlap;
+  unsigned long long selected;
+  long long total_local_gain;
+  unsigned long long validation_row_entry_work;
+  unsigned long long merges;
+  unsigned long long member_rows;              // rows read by the group kernel
+  unsigned long long member_entries;           // cross entries of those rows
+};
+
+}  // namespace beam

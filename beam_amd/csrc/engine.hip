@@ -1,0 +1,1203 @@
+// Host orchestration of the device-resident merge path and its C-ABI.
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/beam_hip.h"
+#include "engine.hpp"
+#include "prims.hpp"
+#include "score_pair.hpp"
+
+namespace beam {
+
+namespace {
+
+thread_local std::string g_error;
+
+#define HIP_CHECK(expr)                                                                     \
+  do {                                                                                      \
+    hipError_t _e = (expr);                                                                 \
+    if (_e != hipSuccess) {                                                                 \
+      throw std::runtime_error(std::string(#expr) + " failed: " + hipGetErrorString(_e));   \
+    }                                                                                       \
+  } while (0)
+
+constexpr int kMaxIterations = 4096;
+constexpr int kCommitGrid = 512;
+constexpr int kRewriteGrid = 512;
+
+template <typename T>
+T* DevAlloc(size_t count) {
+  void* p = nullptr;
+  if (count == 0) count = 1;
+  HIP_CHECK(hipMalloc(&p, count * sizeof(T)));
+  return static_cast<T*>(p);
+}
+
+int Blocks(long long n, int threads) { return static_cast<int>(std::max<long long>(1, (n + threads - 1) / threads)); }
+
+// ---------------------------------------------------------------- kernels
+
+__global__ void InitSingletons(int n, const int64_t* row_ptr, uint32_t* row_off, int32_t* row_len,
+                               int32_t* row_cap, int32_t* size, int32_t* internal, unsigned char* active,
+                               int32_t* parent, int32_t* label) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  row_off[r] = static_cast<uint32_t>(row_ptr[r]);
+  row_len[r] = 0;
+  row_cap[r] = static_cast<int32_t>(row_ptr[r + 1] - row_ptr[r]);
+  size[r] = 1;
+  internal[r] = 0;
+  active[r] = 1;
+  parent[r] = r;
+  label[r] = r;
+}
+
+// Singleton rows from the sorted CSR: each neighbour once with its
+// multiplicity (QuotientGraph::Rebuild under identity labels,
+// quotient_graph.cpp:106-140).
+__global__ void BuildSingletonRows(int n, const int64_t* row_ptr, const int32_t* col, Entry* pool,
+                                   const uint32_t* row_off, int32_t* row_len) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int64_t b = row_ptr[r], e = row_ptr[r + 1];
+  int out = 0;
+  Entry* dst = pool + row_off[r];
+  for (int64_t k = b; k < e; ++k) {
+    const int v = col[k];
+    if (out > 0 && dst[out - 1].nbr == v) {
+      dst[out - 1].cnt += 1;
+    } else {
+      dst[out].nbr = v;
+      dst[out].cnt = 1;
+      ++out;
+    }
+  }
+  row_len[r] = out;
+}
+
+// Resolves node labels through the merge forest; a chain longer than the
+// number of merges possible since the last resolution means a corrupt parent
+// array and raises kErrSelection instead of looping.
+__global__ void Relabel(int n, int32_t* label, const int32_t* parent, int* err) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  int r = label[v];
+  for (int steps = 0; parent[r] != r; ++steps) {
+    if (steps > 4096) {
+      atomicOr(err, kErrSelection);
+      return;
+    }
+    r = parent[r];
+  }
+  label[v] = r;
+}
+
+__global__ void ActiveFlags(int n, const unsigned char* active, uint32_t* flag) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n) flag[r] = active[r] ? 1U : 0U;
+}
+
+__global__ void ScatterFlagged(int n, const uint32_t* flag, const uint32_t* pos, int32_t* out) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n && flag[r]) out[pos[r]] = r;
+}
+
+__global__ void InitOpen(int A, const int32_t* order, uint32_t* open_pos, uint32_t* seg, unsigned char* open_rep) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A) return;
+  open_pos[i] = static_cast<uint32_t>(i);
+  seg[i] = 0;
+  open_rep[order[i]] = 1;
+}
+
+__global__ void ResetMinhash(int K, const uint32_t* open_pos, const int32_t* order, unsigned long long* mh) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < K) mh[order[open_pos[i]]] = ~0ULL;
+}
+
+// Min over each open supernode's members of the node shingle: the minimum
+// HashRank64 over the node and its original neighbours (sweg.cpp:754-784).
+__global__ void NodeShingles(int n, const int64_t* row_ptr, const int32_t* col, const int32_t* label,
+                             const unsigned char* open_rep, unsigned long long* mh, uint64_t seed, int dim,
+                             int iter) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  const int r = label[v];
+  if (!open_rep[r]) return;
+  uint64_t best = HashRank64(seed, v, dim, iter);
+  for (int64_t k = row_ptr[v]; k < row_ptr[v + 1]; ++k) {
+    const uint64_t h = HashRank64(seed, col[k], dim, iter);
+    best = h < best ? h : best;
+  }
+  atomicMin(&mh[r], static_cast<unsigned long long>(best));
+}
+
+__global__ void MakeKeys(int K, const uint32_t* open_pos, const uint32_t* seg, const int32_t* order,
+                         const unsigned long long* mh, SortRec* recs) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= K) return;
+  const int rep = order[open_pos[i]];
+  recs[i].key = mh[rep];
+  recs[i].seg = seg[i];
+  recs[i].val = rep;
+}
+
+__global__ void MakeFallbackKeys(int K, const uint32_t* open_pos, const uint32_t* seg, const int32_t* order,
+                                 SortRec* recs, uint64_t seed, int iter) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= K) return;
+  const int rep = order[open_pos[i]];
+  recs[i].key = FallbackRank(seed, rep, iter);
+  recs[i].seg = seg[i];
+  recs[i].val = rep;
+}
+
+__global__ void WriteBack(int K, const SortRec* recs, const uint32_t* open_pos, int32_t* order) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < K) order[open_pos[i]] = recs[i].val;
+}
+
+__global__ void RunBoundaries(int K, const SortRec* recs, uint32_t* flag) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= K) return;
+  flag[i] = (i == 0 || recs[i].seg != recs[i - 1].seg || recs[i].key != recs[i - 1].key) ? 1U : 0U;
+}
+
+__global__ void SegBoundaries(int K, const SortRec* recs, uint32_t* flag) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= K) return;
+  flag[i] = (i == 0 || recs[i].seg != recs[i - 1].seg) ? 1U : 0U;
+}
+
+__global__ void RunStarts(int K, const uint32_t* flag, const uint32_t* excl, uint32_t* run_start, uint32_t* rid) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= K) return;
+  if (flag[i]) run_start[excl[i]] = static_cast<uint32_t>(i);
+  rid[i] = excl[i] + flag[i] - 1;
+}
+
+// Runs of equal hash become groups when small enough (sweg.cpp:818-822),
+// otherwise open segments of the next depth.
+__global__ void ClassifyRuns(int R, int K, const uint32_t* run_start, const uint32_t* open_pos,
+                             const SortRec* recs, int max_group, uint32_t* gflag, unsigned char* open_rep,
+                             uint32_t* run_open, unsigned long long* max_len) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= R) return;
+  const uint32_t s = run_start[r];
+  const uint32_t e = r + 1 < R ? run_start[r + 1] : static_cast<uint32_t>(K);
+  const uint32_t len = e - s;
+  if (static_cast<int>(len) <= max_group) {
+    gflag[open_pos[s]] = 1;
+    for (uint32_t j = s; j < e; ++j) open_rep[recs[j].val] = 0;
+    run_open[r] = 0;
+    atomicMax(max_len, static_cast<unsigned long long>(len));
+  } else {
+    run_open[r] = 1;
+  }
+}
+
+__global__ void ElemOpen(int K, const uint32_t* rid, const uint32_t* run_open, uint32_t* elem_open) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < K) elem_open[i] = run_open[rid[i]];
+}
+
+__global__ void NewOpen(int K, const uint32_t* elem_open, const uint32_t* new_idx, const uint32_t* rid,
+                        const uint32_t* run_seg, const uint32_t* open_pos, uint32_t* open_pos2, uint32_t* seg2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= K || !elem_open[i]) return;
+  open_pos2[new_idx[i]] = open_pos[i];
+  seg2[new_idx[i]] = run_seg[rid[i]];
+}
+
+// Fallback split (sweg.cpp:786-810): chunks of max_group within each segment.
+__global__ void FallbackChunks(int K, const uint32_t* seg_flag, const uint32_t* seg_excl, const uint32_t* seg_first,
+                               const uint32_t* open_pos, int max_group, uint32_t* gflag, unsigned long long* max_len,
+                               int S) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= K) return;
+  const uint32_t sid = seg_excl[i] + seg_flag[i] - 1;
+  const uint32_t first = seg_first[sid];
+  const uint32_t end = sid + 1 < static_cast<uint32_t>(S) ? seg_first[sid + 1] : static_cast<uint32_t>(K);
+  const uint32_t local = static_cast<uint32_t>(i) - first;
+  if (local % static_cast<uint32_t>(max_group) == 0) {
+    gflag[open_pos[i]] = 1;
+    const uint32_t len = min(static_cast<uint32_t>(max_group), end - static_cast<uint32_t>(i));
+    atomicMax(max_len, static_cast<unsigned long long>(len));
+  }
+}
+
+__global__ void GroupLens(int G, int A, int32_t* gstart, int32_t* glen) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= G) return;
+  const int end = j + 1 < G ? gstart[j + 1] : A;
+  glen[j] = end - gstart[j];
+}
+
+__global__ void FillU32(uint32_t* p, int n, uint32_t v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void FillU8(unsigned char* p, int n, unsigned char v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void ActiveRowLens(int n, const unsigned char* active, const int32_t* row_len, uint32_t* out) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n) out[r] = active[r] ? static_cast<uint32_t>(row_len[r]) : 0U;
+}
+
+// Repacks active rows contiguously into the other pool buffer.
+__global__ void CompactRows(int n, const unsigned char* active, const Entry* src, Entry* dst, uint32_t* row_off,
+                            int32_t* row_len, int32_t* row_cap, const uint32_t* new_off) {
+  const int r = blockIdx.x;
+  if (r >= n || !active[r]) {
+    if (r < n && threadIdx.x == 0) {
+      row_len[r] = 0;
+      row_cap[r] = 0;
+      row_off[r] = new_off[r];
+    }
+    return;
+  }
+  const int len = row_len[r];
+  const Entry* s = src + row_off[r];
+  Entry* d = dst + new_off[r];
+  for (int k = threadIdx.x; k < len; k += blockDim.x) d[k] = s[k];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    row_off[r] = new_off[r];
+    row_cap[r] = len;
+  }
+}
+
+// Block costs of the partition and the MAGS encoding decision per block
+// (sweg.cpp:3104-3151): sparse blocks contribute e corrections (Cp), dense
+// blocks one superedge (P) and cap - e corrections (Cm).
+__global__ void EncodeCounts(int n, const unsigned char* active, const Entry* pool, const uint32_t* row_off,
+                             const int32_t* row_len, const int32_t* size, const int32_t* internal,
+                             unsigned long long* out, int* err) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= n || !active[a]) return;
+  unsigned long long P = 0, Cp = 0, Cm = 0, loops = 0, cost = 0;
+  auto block = [&](int64_t sa, int64_t sb, int64_t e, bool internal_block) {
+    if (e <= 0) return;
+    const int64_t cap = internal_block ? sa * (sa - 1) / 2 : sa * sb;
+    if (e > cap) atomicOr(err, kErrCapacity);
+    if (e <= 1 + cap - e) {
+      Cp += static_cast<unsigned long long>(e);
+      cost += static_cast<unsigned long long>(e);
+    } else {
+      P += 1;
+      if (internal_block) loops += 1;
+      Cm += static_cast<unsigned long long>(cap - e);
+      cost += static_cast<unsigned long long>(1 + cap - e);
+    }
+  };
+  const int64_t sa = size[a];
+  block(sa, sa, internal[a], true);
+  const Entry* row = pool + row_off[a];
+  for (int k = 0; k < row_len[a]; ++k) {
+    if (row[k].nbr > a) block(sa, size[row[k].nbr], row[k].cnt, false);
+  }
+  atomicAdd(&out[0], P);
+  atomicAdd(&out[1], Cp);
+  atomicAdd(&out[2], Cm);
+  atomicAdd(&out[3], loops);
+  atomicAdd(&out[4], 1ULL);
+  atomicAdd(&out[5], cost);
+}
+
+__global__ __launch_bounds__(256) void ScorePairsKernel(QuotientView q, const int32_t* a, const int32_t* b,
+                                                       long long count, long long* gain, long long* before,
+                                                       int* err) {
+  __shared__ int32_t tables[4 * kScoreSlots * 2];
+  int32_t* tkey = tables + WaveId() * kScoreSlots * 2;
+  int32_t* tcnt = tkey + kScoreSlots;
+  const long long waves = static_cast<long long>(gridDim.x) * 4;
+  for (long long p = static_cast<long long>(blockIdx.x) * 4 + WaveId(); p < count; p += waves) {
+    long long g = 0, bf = 0;
+    ScorePairWave(q, a[p], b[p], tkey, tcnt, err, &g, &bf);
+    if (LaneId() == 0) {
+      gain[p] = g;
+      before[p] = bf;
+    }
+  }
+}
+
+__global__ void MarkCommit(const int32_t* sel, const int32_t* sel_count, int nslots, int epoch, int32_t* mark_dest,
+                           int32_t* dest, int32_t* mark_keep, int32_t* parent) {
+  const int s = blockIdx.x;
+  if (s >= nslots) return;
+  for (int k = threadIdx.x; k < sel_count[s]; k += blockDim.x) {
+    const int keep = sel[2 * (s * kMaxSelect + k)], remove = sel[2 * (s * kMaxSelect + k) + 1];
+    mark_dest[remove] = epoch;
+    dest[remove] = keep;
+    mark_keep[keep] = epoch;
+    parent[remove] = keep;
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- engine
+
+struct Engine {
+  beam_hip_options opt{};
+  int n = 0;
+  long long arcs = 0;
+  hipStream_t stream = nullptr;
+
+  int64_t* row_ptr = nullptr;
+  int32_t* col = nullptr;
+  int32_t* label = nullptr;
+  int32_t* parent = nullptr;
+  unsigned char* active = nullptr;
+
+  Entry* pool[2] = {nullptr, nullptr};
+  int pool_idx = 0;
+  unsigned long long pool_cap = 0;
+  unsigned long long* pool_top = nullptr;
+  uint32_t* row_off = nullptr;
+  int32_t* row_len = nullptr;
+  int32_t* row_cap = nullptr;
+  int32_t* size = nullptr;
+  int32_t* internal = nullptr;
+
+  // Divide
+  int32_t* order = nullptr;
+  uint32_t* open_pos[2] = {nullptr, nullptr};
+  uint32_t* seg[2] = {nullptr, nullptr};
+  unsigned char* open_rep = nullptr;
+  unsigned long long* mh = nullptr;
+  SortRec* recs[2] = {nullptr, nullptr};
+  uint32_t* flag = nullptr;
+  uint32_t* excl = nullptr;
+  uint32_t* rid = nullptr;
+  uint32_t* run_start = nullptr;
+  uint32_t* run_open = nullptr;
+  uint32_t* run_seg = nullptr;
+  uint32_t* elem_open = nullptr;
+  uint32_t* new_idx = nullptr;
+  uint32_t* gflag = nullptr;
+  int32_t* gstart = nullptr;
+  int32_t* glen = nullptr;
+  uint32_t* d_total = nullptr;
+  unsigned long long* d_maxlen = nullptr;
+  ScanTemp scan;
+  SortTemp sort;
+
+  // Merge
+  int32_t* sel = nullptr;
+  int32_t* sel_count = nullptr;
+  int32_t* mark_dest = nullptr;
+  int32_t* dest = nullptr;
+  int32_t* mark_keep = nullptr;
+  int32_t* mark_row = nullptr;
+  int32_t* aff = nullptr;
+  int32_t* naff = nullptr;  // [2]
+  unsigned char* scratch_a = nullptr;
+  unsigned long long scratch_a_cap = 0;
+  unsigned char* scratch_u = nullptr;
+  unsigned long long scratch_u_cap = 0;
+  unsigned long long* tops = nullptr;  // [0] = groups scratch, [1] = union scratch
+  Counters* ctr = nullptr;             // per iteration
+  int* err = nullptr;
+  unsigned long long* enc = nullptr;
+  int epoch = 0;
+  long long blocks_run = 0;
+
+  // Host-side bookkeeping
+  int64_t divide_max_group = 0;
+  int64_t divide_fallback_splits = 0;
+  std::vector<int64_t> round_rows;  // 10 per iteration
+  std::vector<Counters> iter_ctr;
+  double run_ms = 0, divide_ms = 0, merge_ms = 0;
+  int last_groups = 0;
+  int last_active = 0;
+  // Optional per-launch timing of the group kernel (HIP events on the engine
+  // stream); enabled by beam_hip_profile.
+  bool profile = false;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  double group_kernel_ms = 0;
+  long long group_kernel_launches = 0;
+  unsigned long long member_rows = 0, member_entries = 0, pairs_scored = 0;
+
+  ~Engine();
+  void Create(const int64_t* h_row_ptr, const int32_t* h_col, int n_, long long arcs_, const beam_hip_options& o);
+  void Sync() { HIP_CHECK(hipStreamSynchronize(stream)); }
+  void CheckDeviceError(const char* where);
+  int Divide(int iter);
+  void MergeBlocks(int iter, int ngroups, double theta, Counters* c);
+  void LaunchBlock(int first_group, int nslots, int ngroups, double theta, Counters* c, const int32_t* order_arr,
+                   const int32_t* gstart_arr, const int32_t* glen_arr, int32_t* dbg_pairs, long long* dbg_gain,
+                   int32_t* dbg_count, int dbg_cap, bool commit);
+  void Commit(int nslots, Counters* c);
+  void EnsurePoolRoom();
+  void Iterate(int iter);
+  void Reset();
+  void CollectEvents();
+  uint32_t ScanTotal(const uint32_t* in, uint32_t* out, int count);
+  long long ActiveNnz();
+};
+
+Engine::~Engine() {
+  if (stream) hipStreamSynchronize(stream);
+  void* ptrs[] = {row_ptr,  col,      label,    parent,    active,   pool[0],    pool[1],   pool_top, row_off,
+                  row_len,  row_cap,  size,     internal,  order,    open_pos[0], open_pos[1], seg[0], seg[1],
+                  open_rep, mh,       recs[0],  recs[1],   flag,     excl,       rid,       run_start, run_open,
+                  run_seg,  elem_open, new_idx, gflag,     gstart,   glen,       d_total,   d_maxlen, sel,
+                  sel_count, mark_dest, dest,   mark_keep, mark_row, aff,        naff,      scratch_a, scratch_u,
+                  tops,     ctr,      err,      enc};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  for (hipEvent_t ev : ev_pool) (void)hipEventDestroy(ev);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+void Engine::Create(const int64_t* h_row_ptr, const int32_t* h_col, int n_, long long arcs_,
+                    const beam_hip_options& o) {
+  opt = o;
+  if (opt.top_k <= 0) throw std::invalid_argument("top_k must be positive");
+  if (opt.group_batch_size < 1) opt.group_batch_size = 1;
+  if (opt.group_batch_size > kMaxSlotsPerBlock) throw std::invalid_argument("group_batch_size above 1024 is not supported");
+  if (opt.divide_hash_dims <= 0) opt.divide_hash_dims = 16;
+  if (opt.divide_max_group <= 0) opt.divide_max_group = 512;
+  if (opt.divide_max_group > 512) throw std::invalid_argument("divide_max_group above 512 is not supported");
+  if (!opt.ea_use_threshold) throw std::invalid_argument("the HIP backend implements the reciprocal --ea-use-threshold path only");
+  n = n_;
+  arcs = arcs_;
+  if (n <= 0) throw std::invalid_argument("Invalid node count.");
+  if (h_row_ptr[n] != arcs) throw std::invalid_argument("row_ptr[n] must equal the arc count");
+  HIP_CHECK(hipSetDevice(opt.device));
+  HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+
+  row_ptr = DevAlloc<int64_t>(static_cast<size_t>(n) + 1);
+  col = DevAlloc<int32_t>(static_cast<size_t>(arcs));
+  HIP_CHECK(hipMemcpyAsync(row_ptr, h_row_ptr, (static_cast<size_t>(n) + 1) * sizeof(int64_t), hipMemcpyHostToDevice, stream));
+  HIP_CHECK(hipMemcpyAsync(col, h_col, static_cast<size_t>(arcs) * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+  label = DevAlloc<int32_t>(n);
+  parent = DevAlloc<int32_t>(n);
+  active = DevAlloc<unsigned char>(n);
+  // Row storage: the live rows never exceed the initial arcs; each
+  // iteration's unions need at most that much fresh space, so two and a half
+  // times the arcs with compaction between iterations always suffices.
+  pool_cap = static_cast<unsigned long long>(arcs) * 5 / 2 + 1024;
+  if (pool_cap > 0xffffffffULL) throw std::invalid_argument("graph too large for 32-bit row offsets");
+  pool[0] = DevAlloc<Entry>(pool_cap);
+  pool[1] = DevAlloc<Entry>(pool_cap);
+  pool_top = DevAlloc<unsigned long long>(1);
+  row_off = DevAlloc<uint32_t>(n);
+  row_len = DevAlloc<int32_t>(n);
+  row_cap = DevAlloc<int32_t>(n);
+  size = DevAlloc<int32_t>(n);
+  internal = DevAlloc<int32_t>(n);
+
+  order = DevAlloc<int32_t>(n);
+  for (int i = 0; i < 2; ++i) {
+    open_pos[i] = DevAlloc<uint32_t>(n);
+    seg[i] = DevAlloc<uint32_t>(n);
+    recs[i] = DevAlloc<SortRec>(n);
+  }
+  open_rep = DevAlloc<unsigned char>(n);
+  HIP_CHECK(hipMemsetAsync(open_rep, 0, n, stream));
+  mh = DevAlloc<unsigned long long>(n);
+  flag = DevAlloc<uint32_t>(static_cast<size_t>(n) + 1);
+  excl = DevAlloc<uint32_t>(static_cast<size_t>(n) + 1);
+  rid = DevAlloc<uint32_t>(n);
+  run_start = DevAlloc<uint32_t>(n);
+  run_open = DevAlloc<uint32_t>(static_cast<size_t>(n) + 1);
+  run_seg = DevAlloc<uint32_t>(static_cast<size_t>(n) + 1);
+  elem_open = DevAlloc<uint32_t>(static_cast<size_t>(n) + 1);
+  new_idx = DevAlloc<uint32_t>(static_cast<size_t>(n) + 1);
+  gflag = DevAlloc<uint32_t>(static_cast<size_t>(n) + 1);
+  gstart = DevAlloc<int32_t>(static_cast<size_t>(n) + 1);
+  glen = DevAlloc<int32_t>(static_cast<size_t>(n) + 1);
+  d_total = DevAlloc<uint32_t>(4);
+  d_maxlen = DevAlloc<unsigned long long>(1);
+
+  sel = DevAlloc<int32_t>(static_cast<size_t>(kMaxSlotsPerBlock) * kMaxSelect * 2);
+  sel_count = DevAlloc<int32_t>(kMaxSlotsPerBlock);
+  mark_dest = DevAlloc<int32_t>(n);
+  dest = DevAlloc<int32_t>(n);
+  mark_keep = DevAlloc<int32_t>(n);
+  mark_row = DevAlloc<int32_t>(n);
+  aff = DevAlloc<int32_t>(n);
+  naff = DevAlloc<int32_t>(2);
+  HIP_CHECK(hipMemsetAsync(naff, 0, 2 * sizeof(int32_t), stream));
+  const unsigned long long work = static_cast<unsigned long long>(arcs) + static_cast<unsigned long long>(n);
+  scratch_a_cap = std::min<unsigned long long>(std::max<unsigned long long>(40ULL * work + (128ULL << 20), 256ULL << 20), 8ULL << 30);
+  scratch_u_cap = std::min<unsigned long long>(std::max<unsigned long long>(16ULL * work + (64ULL << 20), 128ULL << 20), 4ULL << 30);
+  scratch_a = DevAlloc<unsigned char>(scratch_a_cap);
+  scratch_u = DevAlloc<unsigned char>(scratch_u_cap);
+  tops = DevAlloc<unsigned long long>(2);
+  HIP_CHECK(hipMemsetAsync(tops, 0, 2 * sizeof(unsigned long long), stream));
+  ctr = DevAlloc<Counters>(kMaxIterations + 1);
+  HIP_CHECK(hipMemsetAsync(ctr, 0, sizeof(Counters) * (kMaxIterations + 1), stream));
+  err = DevAlloc<int>(1);
+  HIP_CHECK(hipMemsetAsync(err, 0, sizeof(int), stream));
+  enc = DevAlloc<unsigned long long>(8);
+  Reset();
+}
+
+// Singleton partition and its quotient from the resident CSR (Sweg::Sweg,
+// sweg.cpp:102-194); clears every mark and counter.
+void Engine::Reset() {
+  pool_idx = 0;
+  InitSingletons<<<Blocks(n, 256), 256, 0, stream>>>(n, row_ptr, row_off, row_len, row_cap, size, internal, active,
+                                                     parent, label);
+  BuildSingletonRows<<<Blocks(n, 256), 256, 0, stream>>>(n, row_ptr, col, pool[0], row_off, row_len);
+  const unsigned long long top = static_cast<unsigned long long>(arcs);
+  HIP_CHECK(hipMemcpyAsync(pool_top, &top, sizeof(top), hipMemcpyHostToDevice, stream));
+  HIP_CHECK(hipMemsetAsync(mark_dest, 0, sizeof(int32_t) * n, stream));
+  HIP_CHECK(hipMemsetAsync(mark_keep, 0, sizeof(int32_t) * n, stream));
+  HIP_CHECK(hipMemsetAsync(mark_row, 0, sizeof(int32_t) * n, stream));
+  HIP_CHECK(hipMemsetAsync(naff, 0, 2 * sizeof(int32_t), stream));
+  HIP_CHECK(hipMemsetAsync(tops, 0, 2 * sizeof(unsigned long long), stream));
+  HIP_CHECK(hipMemsetAsync(ctr, 0, sizeof(Counters) * (kMaxIterations + 1), stream));
+  HIP_CHECK(hipMemsetAsync(err, 0, sizeof(int), stream));
+  epoch = 0;
+  blocks_run = 0;
+  divide_max_group = 0;
+  divide_fallback_splits = 0;
+  round_rows.clear();
+  iter_ctr.clear();
+  run_ms = divide_ms = merge_ms = 0;
+  group_kernel_ms = 0;
+  group_kernel_launches = 0;
+  member_rows = member_entries = pairs_scored = 0;
+  Sync();
+  CheckDeviceError("reset");
+}
+
+void Engine::CollectEvents() {
+  for (size_t i = 0; i + 1 < ev_used; i += 2) {
+    float ms = 0;
+    HIP_CHECK(hipEventElapsedTime(&ms, ev_pool[i], ev_pool[i + 1]));
+    group_kernel_ms += ms;
+    ++group_kernel_launches;
+  }
+  ev_used = 0;
+}
+
+void Engine::CheckDeviceError(const char* where) {
+  int e = 0;
+  HIP_CHECK(hipMemcpy(&e, err, sizeof(int), hipMemcpyDeviceToHost));
+  if (e == 0) return;
+  std::string msg = std::string(where) + ": device error:";
+  if (e & kErrCapacity) msg += " edge count exceeds block capacity;";
+  if (e & kErrAsymmetric) msg += " asymmetric quotient edge count during scoring;";
+  if (e & kErrScratch) msg += " per-block scratch exhausted;";
+  if (e & kErrPool) msg += " quotient entry pool exhausted;";
+  if (e & kErrSpecials) msg += " too many merge endpoints in one row;";
+  if (e & kErrSelection) msg += " group larger than 512;";
+  throw std::runtime_error(msg);
+}
+
+uint32_t Engine::ScanTotal(const uint32_t* in, uint32_t* out, int count) {
+  DeviceScan(in, out, count, d_total, scan, stream);
+  uint32_t total = 0;
+  HIP_CHECK(hipMemcpyAsync(&total, d_total, sizeof(total), hipMemcpyDeviceToHost, stream));
+  Sync();
+  return total;
+}
+
+long long Engine::ActiveNnz() {
+  ActiveRowLens<<<Blocks(n, 256), 256, 0, stream>>>(n, active, row_len, flag);
+  return static_cast<long long>(ScanTotal(flag, excl, n));
+}
+
+// Divide (sweg.cpp:853-890), level-synchronous: at each depth every open
+// segment is stably sorted by (segment, min-hash) -- within a segment the
+// members are already in rep order, so this is the reference's (hash, rep)
+// order -- and split into runs of equal hash.
+int Engine::Divide(int iter) {
+  Relabel<<<Blocks(n, 256), 256, 0, stream>>>(n, label, parent, err);
+  ActiveFlags<<<Blocks(n, 256), 256, 0, stream>>>(n, active, flag);
+  const int A = static_cast<int>(ScanTotal(flag, excl, n));
+  ScatterFlagged<<<Blocks(n, 256), 256, 0, stream>>>(n, flag, excl, order);
+  last_active = A;
+  // dim order by DimRank (sweg.cpp:739-744, 867-876), computed on the host.
+  std::vector<int> dims(static_cast<size_t>(opt.divide_hash_dims));
+  std::iota(dims.begin(), dims.end(), 0);
+  auto dim_rank = [&](int dim) {
+    uint64_t x = opt.seed;
+    x ^= 0x9e3779b97f4a7c15ULL * static_cast<uint64_t>(dim + 1);
+    x ^= 0xbf58476d1ce4e5b9ULL * static_cast<uint64_t>(iter + 1);
+    x += 0x9e3779b97f4a7c15ULL;
+    x = (x ^ (x >> 30U)) * 0xbf58476d1ce4e5b9ULL;
+    x = (x ^ (x >> 27U)) * 0x94d049bb133111ebULL;
+    return x ^ (x >> 31U);
+  };
+  std::stable_sort(dims.begin(), dims.end(), [&](int a, int b) {
+    const uint64_t ra = dim_rank(a), rb = dim_rank(b);
+    return ra != rb ? ra < rb : a < b;
+  });
+  FillU32<<<Blocks(A + 1, 256), 256, 0, stream>>>(gflag, A + 1, 0U);
+  HIP_CHECK(hipMemsetAsync(d_maxlen, 0, sizeof(unsigned long long), stream));
+  int K = 0;
+  int cur = 0;
+  if (A > opt.divide_max_group) {
+    InitOpen<<<Blocks(A, 256), 256, 0, stream>>>(A, order, open_pos[0], seg[0], open_rep);
+    K = A;
+  } else if (A > 0) {
+    FillU32<<<1, 1, 0, stream>>>(gflag, 1, 1U);
+    const unsigned long long len = static_cast<unsigned long long>(A);
+    HIP_CHECK(hipMemcpyAsync(d_maxlen, &len, sizeof(len), hipMemcpyHostToDevice, stream));
+  }
+  int depth = 0;
+  while (K > 0 && depth < opt.divide_hash_dims) {
+    const int dim = dims[static_cast<size_t>(depth)];
+    ResetMinhash<<<Blocks(K, 256), 256, 0, stream>>>(K, open_pos[cur], order, mh);
+    NodeShingles<<<Blocks(n, 256), 256, 0, stream>>>(n, row_ptr, col, label, open_rep, mh, opt.seed, dim, iter);
+    MakeKeys<<<Blocks(K, 256), 256, 0, stream>>>(K, open_pos[cur], seg[cur], order, mh, recs[0]);
+    SortRec* sorted = SegmentedKeySort(recs[0], recs[1], K, sort, stream);
+    WriteBack<<<Blocks(K, 256), 256, 0, stream>>>(K, sorted, open_pos[cur], order);
+    RunBoundaries<<<Blocks(K, 256), 256, 0, stream>>>(K, sorted, flag);
+    const int R = static_cast<int>(ScanTotal(flag, excl, K));
+    RunStarts<<<Blocks(K, 256), 256, 0, stream>>>(K, flag, excl, run_start, rid);
+    ClassifyRuns<<<Blocks(R, 256), 256, 0, stream>>>(R, K, run_start, open_pos[cur], sorted, opt.divide_max_group,
+                                                    gflag, open_rep, run_open, d_maxlen);
+    DeviceScan(run_open, run_seg, R, d_total, scan, stream);
+    ElemOpen<<<Blocks(K, 256), 256, 0, stream>>>(K, rid, run_open, elem_open);
+    const int K2 = static_cast<int>(ScanTotal(elem_open, new_idx, K));
+    NewOpen<<<Blocks(K, 256), 256, 0, stream>>>(K, elem_open, new_idx, rid, run_seg, open_pos[cur], open_pos[1 - cur],
+                                               seg[1 - cur]);
+    cur = 1 - cur;
+    K = K2;
+    ++depth;
+  }
+  if (K > 0) {
+    MakeFallbackKeys<<<Blocks(K, 256), 256, 0, stream>>>(K, open_pos[cur], seg[cur], order, recs[0], opt.seed, iter);
+    SortRec* sorted = SegmentedKeySort(recs[0], recs[1], K, sort, stream);
+    WriteBack<<<Blocks(K, 256), 256, 0, stream>>>(K, sorted, open_pos[cur], order);
+    SegBoundaries<<<Blocks(K, 256), 256, 0, stream>>>(K, sorted, flag);
+    const int S = static_cast<int>(ScanTotal(flag, excl, K));
+    RunStarts<<<Blocks(K, 256), 256, 0, stream>>>(K, flag, excl, run_start, rid);
+    FallbackChunks<<<Blocks(K, 256), 256, 0, stream>>>(K, flag, excl, run_start, open_pos[cur], opt.divide_max_group,
+                                                      gflag, d_maxlen, S);
+    FillU8<<<Blocks(n, 256), 256, 0, stream>>>(open_rep, n, 0);
+    divide_fallback_splits += S;
+  }
+  const int G = static_cast<int>(ScanTotal(gflag, excl, A));
+  ScatterFlagged<<<Blocks(A, 256), 256, 0, stream>>>(A, gflag, excl, gstart);
+  GroupLens<<<Blocks(G, 256), 256, 0, stream>>>(G, A, gstart, glen);
+  unsigned long long maxlen = 0;
+  HIP_CHECK(hipMemcpyAsync(&maxlen, d_maxlen, sizeof(maxlen), hipMemcpyDeviceToHost, stream));
+  Sync();
+  divide_max_group = std::max<int64_t>(divide_max_group, static_cast<int64_t>(maxlen));
+  last_groups = G;
+  return G;
+}
+
+void Engine::LaunchBlock(int first_group, int nslots, int ngroups, double theta, Counters* c, const int32_t* order_arr,
+                         const int32_t* gstart_arr, const int32_t* glen_arr, int32_t* dbg_pairs, long long* dbg_gain,
+                         int32_t* dbg_count, int dbg_cap, bool commit) {
+  ++epoch;
+  MergeParams mp{};
+  mp.q = QuotientView{pool[pool_idx], row_off, row_len, size, internal};
+  mp.order = order_arr;
+  mp.gstart = gstart_arr;
+  mp.glen = glen_arr;
+  mp.ngroups = ngroups;
+  mp.first_group = first_group;
+  mp.top_k = opt.top_k;
+  mp.theta = theta;
+  mp.epoch = epoch;
+  mp.sel = sel;
+  mp.sel_count = sel_count;
+  mp.mark_dest = mark_dest;
+  mp.dest = dest;
+  mp.mark_keep = mark_keep;
+  mp.parent = parent;
+  mp.scratch = scratch_a;
+  mp.scratch_cap = scratch_a_cap;
+  mp.scratch_top = &tops[0];
+  mp.ctr = c;
+  mp.err = err;
+  mp.dbg_pairs = dbg_pairs;
+  mp.dbg_gain = dbg_gain;
+  mp.dbg_count = dbg_count;
+  mp.dbg_cap = dbg_cap;
+  mp.commit = commit ? 1 : 0;
+  if (profile) {
+    if (ev_used + 2 > ev_pool.size()) {
+      for (int k = 0; k < 256; ++k) {
+        hipEvent_t ev;
+        HIP_CHECK(hipEventCreate(&ev));
+        ev_pool.push_back(ev);
+      }
+    }
+    HIP_CHECK(hipEventRecord(ev_pool[ev_used++], stream));
+  }
+  MergeGroupsKernel<<<nslots, 256, 0, stream>>>(mp);
+  if (profile) HIP_CHECK(hipEventRecord(ev_pool[ev_used++], stream));
+  ++blocks_run;
+  if (commit) Commit(nslots, c);
+}
+
+void Engine::Commit(int nslots, Counters* c) {
+  CommitParams cp{};
+  cp.pool = pool[pool_idx];
+  cp.row_off = row_off;
+  cp.row_len = row_len;
+  cp.row_cap = row_cap;
+  cp.size = size;
+  cp.internal = internal;
+  cp.active = active;
+  cp.sel = sel;
+  cp.sel_count = sel_count;
+  cp.nslots = nslots;
+  cp.epoch = epoch;
+  cp.mark_dest = mark_dest;
+  cp.dest = dest;
+  cp.mark_keep = mark_keep;
+  cp.mark_row = mark_row;
+  cp.aff = aff;
+  cp.naff = &naff[epoch & 1];
+  cp.naff_next = &naff[(epoch + 1) & 1];
+  cp.pool_top = pool_top;
+  cp.pool_cap = pool_cap;
+  cp.scratch = scratch_u;
+  cp.scratch_cap = scratch_u_cap;
+  cp.scratch_top = &tops[1];
+  cp.reset_top = &tops[0];
+  cp.ctr = c;
+  cp.err = err;
+  UnionKernel<<<kCommitGrid, 256, 0, stream>>>(cp);
+  cp.reset_top = &tops[1];
+  RewriteKernel<<<kRewriteGrid, 256, 0, stream>>>(cp);
+}
+
+void Engine::EnsurePoolRoom() {
+  unsigned long long top = 0;
+  HIP_CHECK(hipMemcpyAsync(&top, pool_top, sizeof(top), hipMemcpyDeviceToHost, stream));
+  Sync();
+  const long long live = ActiveNnz();
+  if (top + static_cast<unsigned long long>(live) + 1024 <= pool_cap) return;
+  // excl holds the exclusive offsets of the active row lengths.
+  const int other = 1 - pool_idx;
+  CompactRows<<<n, 256, 0, stream>>>(n, active, pool[pool_idx], pool[other], row_off, row_len, row_cap, excl);
+  pool_idx = other;
+  const unsigned long long nt = static_cast<unsigned long long>(live);
+  HIP_CHECK(hipMemcpyAsync(pool_top, &nt, sizeof(nt), hipMemcpyHostToDevice, stream));
+  Sync();
+}
+
+void Engine::Iterate(int iter) {
+  if (iter < 1 || iter > kMaxIterations) throw std::invalid_argument("iteration out of range");
+  const auto t0 = std::chrono::steady_clock::now();
+  const int G = Divide(iter);
+  const auto t1 = std::chrono::steady_clock::now();
+  EnsurePoolRoom();
+  const double theta = 1.0 / static_cast<double>(iter + 1);  // reciprocal policy (sweg.cpp:400-406)
+  Counters* c = ctr + iter;
+  const int bs = opt.group_batch_size;
+  for (int first = 0; first < G; first += bs) {
+    const int nslots = std::min(bs, G - first);
+    LaunchBlock(first, nslots, G, theta, c, order, gstart, glen, nullptr, nullptr, nullptr, 0, true);
+  }
+  Sync();
+  const auto t2 = std::chrono::steady_clock::now();
+  CheckDeviceError("merge");
+  if (profile) CollectEvents();
+  Counters h{};
+  HIP_CHECK(hipMemcpy(&h, c, sizeof(h), hipMemcpyDeviceToHost));
+  iter_ctr.push_back(h);
+  member_rows += h.member_rows;
+  member_entries += h.member_entries;
+  pairs_scored += h.exact_gain_calls;
+  const int64_t row[10] = {iter,
+                           last_active,
+                           G,
+                           static_cast<int64_t>(h.proxy_pairs_examined),
+                           static_cast<int64_t>(h.candidate_pairs_after_prune),
+                           static_cast<int64_t>(h.exact_gain_calls),
+                           static_cast<int64_t>(h.positive_gain_count),
+                           static_cast<int64_t>(h.above_threshold),
+                           static_cast<int64_t>(h.selected),
+                           static_cast<int64_t>(h.merges)};
+  round_rows.insert(round_rows.end(), row, row + 10);
+  divide_ms += std::chrono::duration<double, std::milli>(t1 - t0).count();
+  merge_ms += std::chrono::duration<double, std::milli>(t2 - t1).count();
+}
+
+}  // namespace beam
+
+// ---------------------------------------------------------------- C-ABI
+
+struct beam_hip_engine {
+  beam::Engine e;
+};
+
+namespace {
+
+template <typename F>
+int Guard(F&& f) {
+  try {
+    return f();
+  } catch (const std::exception& ex) {
+    beam::g_error = ex.what();
+    return -1;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* beam_hip_build_mode(void) { return "hip"; }
+
+int beam_hip_compiled(void) { return 1; }
+
+const char* beam_hip_last_error(void) { return beam::g_error.c_str(); }
+
+int beam_hip_create(const int64_t* row_ptr, const int32_t* col_idx, int32_t n, int64_t arcs,
+                    const beam_hip_options* options, beam_hip_engine** engine) {
+  return Guard([&] {
+    if (engine == nullptr || options == nullptr || row_ptr == nullptr || (arcs > 0 && col_idx == nullptr)) {
+      throw std::invalid_argument("beam_hip_create: null argument");
+    }
+    auto* h = new beam_hip_engine;
+    try {
+      h->e.Create(row_ptr, col_idx, n, arcs, *options);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *engine = h;
+    return 0;
+  });
+}
+
+void beam_hip_destroy(beam_hip_engine* engine) { delete engine; }
+
+int beam_hip_iterate(beam_hip_engine* engine, int32_t iter) {
+  return Guard([&] {
+    engine->e.Iterate(iter);
+    return 0;
+  });
+}
+
+int beam_hip_run(beam_hip_engine* engine, int32_t iterations) {
+  return Guard([&] {
+    auto& e = engine->e;
+    e.run_ms = e.divide_ms = e.merge_ms = 0;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int it = 1; it <= iterations; ++it) e.Iterate(it);
+    e.run_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return 0;
+  });
+}
+
+int beam_hip_labels(beam_hip_engine* engine, int32_t* labels) {
+  return Guard([&] {
+    auto& e = engine->e;
+    beam::Relabel<<<beam::Blocks(e.n, 256), 256, 0, e.stream>>>(e.n, e.label, e.parent, e.err);
+    HIP_CHECK(hipMemcpyAsync(labels, e.label, sizeof(int32_t) * e.n, hipMemcpyDeviceToHost, e.stream));
+    e.Sync();
+    return 0;
+  });
+}
+
+int beam_hip_stats(beam_hip_engine* engine, int64_t* out) {
+  return Guard([&] {
+    auto& e = engine->e;
+    beam::Counters t{};
+    for (const auto& c : e.iter_ctr) {
+      t.group_count += c.group_count;
+      t.group_max_size = std::max(t.group_max_size, c.group_max_size);
+      t.raw_pair_count += c.raw_pair_count;
+      t.candidate_pairs_after_prune += c.candidate_pairs_after_prune;
+      t.exact_gain_calls += c.exact_gain_calls;
+      t.above_threshold += c.above_threshold;
+      t.rejected_by_threshold += c.rejected_by_threshold;
+      t.rejected_by_overlap += c.rejected_by_overlap;
+      t.selected += c.selected;
+      t.total_local_gain += c.total_local_gain;
+      t.validation_row_entry_work += c.validation_row_entry_work;
+      t.merges += c.merges;
+    }
+    out[BEAM_HIP_STAT_DIVIDE_MAX_GROUP_SIZE] = e.divide_max_group;
+    out[BEAM_HIP_STAT_DIVIDE_FALLBACK_SPLITS] = e.divide_fallback_splits;
+    out[BEAM_HIP_STAT_MERGE_CANDIDATE_PAIRS] = static_cast<int64_t>(t.above_threshold);
+    out[BEAM_HIP_STAT_MERGE_SELECTED_PAIRS] = static_cast<int64_t>(t.selected);
+    out[BEAM_HIP_STAT_MERGE_TOTAL_LOCAL_GAIN] = t.total_local_gain;
+    out[BEAM_HIP_STAT_MERGE_GROUP_COUNT] = static_cast<int64_t>(t.group_count);
+    out[BEAM_HIP_STAT_MERGE_GROUP_MAX_SIZE] = static_cast<int64_t>(t.group_max_size);
+    out[BEAM_HIP_STAT_MERGE_RAW_PAIR_COUNT] = static_cast<int64_t>(t.raw_pair_count);
+    out[BEAM_HIP_STAT_MERGE_CANDIDATE_PAIRS_AFTER_PRUNE] = static_cast<int64_t>(t.candidate_pairs_after_prune);
+    out[BEAM_HIP_STAT_MERGE_EXACT_GAIN_CALLS] = static_cast<int64_t>(t.exact_gain_calls);
+    out[BEAM_HIP_STAT_MERGE_POSITIVE_GAIN_PAIRS] = static_cast<int64_t>(t.above_threshold);
+    out[BEAM_HIP_STAT_MERGE_REJECTED_BY_OVERLAP] = static_cast<int64_t>(t.rejected_by_overlap);
+    out[BEAM_HIP_STAT_MERGE_REJECTED_BY_THRESHOLD] = static_cast<int64_t>(t.rejected_by_threshold);
+    // Validation re-scores each selected pair on the block-start state, so it
+    // agrees with the scored gain (> 0) and accepts every selected pair
+    // (sweg.cpp:2683-2719).
+    out[BEAM_HIP_STAT_SELECTED_MERGES_FOR_VALIDATION] = static_cast<int64_t>(t.selected);
+    out[BEAM_HIP_STAT_ACCEPTED_MERGES_AFTER_VALIDATION] = static_cast<int64_t>(t.merges);
+    out[BEAM_HIP_STAT_REJECTED_NONPOSITIVE] = static_cast<int64_t>(t.selected - t.merges);
+    out[BEAM_HIP_STAT_VALIDATION_EXACT_CALLS] = static_cast<int64_t>(t.selected);
+    out[BEAM_HIP_STAT_VALIDATION_EXACT_ROW_ENTRY_WORK] = static_cast<int64_t>(t.validation_row_entry_work);
+    out[BEAM_HIP_STAT_ISOLATED_GAIN_SUM] = t.total_local_gain;
+    out[BEAM_HIP_STAT_BLOCKS] = e.blocks_run;
+    return 0;
+  });
+}
+
+int beam_hip_round_stats(beam_hip_engine* engine, int64_t* out, int32_t max_rounds) {
+  return Guard([&] {
+    const auto& rows = engine->e.round_rows;
+    const int count = std::min<int>(max_rounds, static_cast<int>(rows.size() / 10));
+    std::copy(rows.begin(), rows.begin() + 10 * count, out);
+    return count;
+  });
+}
+
+int beam_hip_timing(beam_hip_engine* engine, double* out) {
+  out[0] = engine->e.run_ms;
+  out[1] = engine->e.divide_ms;
+  out[2] = engine->e.merge_ms;
+  return 0;
+}
+
+int beam_hip_reset(beam_hip_engine* engine) {
+  return Guard([&] {
+    engine->e.Reset();
+    return 0;
+  });
+}
+
+int beam_hip_profile(beam_hip_engine* engine, int32_t enable) {
+  engine->e.profile = enable != 0;
+  return 0;
+}
+
+int beam_hip_kernel_profile(beam_hip_engine* engine, double* out) {
+  const auto& e = engine->e;
+  out[0] = e.group_kernel_ms;
+  out[1] = static_cast<double>(e.group_kernel_launches);
+  out[2] = static_cast<double>(e.member_rows);
+  out[3] = static_cast<double>(e.member_entries);
+  out[4] = static_cast<double>(e.pairs_scored);
+  return 0;
+}
+
+uint64_t beam_hip_partition_hash(const int32_t* labels, int32_t n) {
+  uint64_t h = 1469598103934665603ULL;  // FNV-1a 64 (sweg.cpp:2920-2928)
+  for (int32_t v = 0; v < n; ++v) {
+    h ^= static_cast<uint64_t>(static_cast<uint32_t>(labels[v]));
+    h *= 1099511628211ULL;
+  }
+  return h;
+}
+
+int beam_hip_encoding_counts(beam_hip_engine* engine, int64_t* out) {
+  return Guard([&] {
+    auto& e = engine->e;
+    HIP_CHECK(hipMemsetAsync(e.enc, 0, 8 * sizeof(unsigned long long), e.stream));
+    beam::EncodeCounts<<<beam::Blocks(e.n, 256), 256, 0, e.stream>>>(e.n, e.active, e.pool[e.pool_idx], e.row_off,
+                                                                     e.row_len, e.size, e.internal, e.enc, e.err);
+    unsigned long long h[8];
+    HIP_CHECK(hipMemcpyAsync(h, e.enc, sizeof(h), hipMemcpyDeviceToHost, e.stream));
+    e.Sync();
+    e.CheckDeviceError("encode");
+    for (int i = 0; i < 6; ++i) out[i] = static_cast<int64_t>(h[i]);
+    return 0;
+  });
+}
+
+int64_t beam_hip_nnz(beam_hip_engine* engine) {
+  int64_t v = -1;
+  Guard([&] {
+    v = engine->e.ActiveNnz();
+    return 0;
+  });
+  return v;
+}
+
+int beam_hip_upload_state(beam_hip_engine* engine, const int64_t* offsets, const int32_t* nbr, const int32_t* cnt,
+                          const int32_t* sizes, const int32_t* internal, const uint8_t* active,
+                          const int32_t* labels) {
+  return Guard([&] {
+    auto& e = engine->e;
+    const int n = e.n;
+    const long long total = offsets[n];
+    if (static_cast<unsigned long long>(total) > e.pool_cap) throw std::invalid_argument("state exceeds the entry pool");
+    std::vector<beam::Entry> entries(static_cast<size_t>(total));
+    for (long long k = 0; k < total; ++k) entries[static_cast<size_t>(k)] = beam::Entry{nbr[k], cnt[k]};
+    std::vector<uint32_t> off(static_cast<size_t>(n));
+    std::vector<int32_t> len(static_cast<size_t>(n)), parent(static_cast<size_t>(n));
+    for (int r = 0; r < n; ++r) {
+      off[static_cast<size_t>(r)] = static_cast<uint32_t>(offsets[r]);
+      len[static_cast<size_t>(r)] = static_cast<int32_t>(offsets[r + 1] - offsets[r]);
+      parent[static_cast<size_t>(r)] = r;
+    }
+    for (int v = 0; v < n; ++v) {
+      if (labels[v] != v) parent[static_cast<size_t>(v)] = labels[v];
+    }
+    e.pool_idx = 0;
+    HIP_CHECK(hipMemcpyAsync(e.pool[0], entries.data(), entries.size() * sizeof(beam::Entry), hipMemcpyHostToDevice, e.stream));
+    HIP_CHECK(hipMemcpyAsync(e.row_off, off.data(), off.size() * sizeof(uint32_t), hipMemcpyHostToDevice, e.stream));
+    HIP_CHECK(hipMemcpyAsync(e.row_len, len.data(), len.size() * sizeof(int32_t), hipMemcpyHostToDevice, e.stream));
+    HIP_CHECK(hipMemcpyAsync(e.row_cap, len.data(), len.size() * sizeof(int32_t), hipMemcpyHostToDevice, e.stream));
+    HIP_CHECK(hipMemcpyAsync(e.size, sizes, sizeof(int32_t) * n, hipMemcpyHostToDevice, e.stream));
+    HIP_CHECK(hipMemcpyAsync(e.internal, internal, sizeof(int32_t) * n, hipMemcpyHostToDevice, e.stream));
+    HIP_CHECK(hipMemcpyAsync(e.active, active, n, hipMemcpyHostToDevice, e.stream));
+    HIP_CHECK(hipMemcpyAsync(e.label, labels, sizeof(int32_t) * n, hipMemcpyHostToDevice, e.stream));
+    HIP_CHECK(hipMemcpyAsync(e.parent, parent.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, e.stream));
+    const unsigned long long top = static_cast<unsigned long long>(total);
+    HIP_CHECK(hipMemcpyAsync(e.pool_top, &top, sizeof(top), hipMemcpyHostToDevice, e.stream));
+    e.Sync();
+    return 0;
+  });
+}
+
+int beam_hip_download_state(beam_hip_engine* engine, int64_t* offsets, int32_t* nbr, int32_t* cnt, int32_t* sizes,
+                            int32_t* internal, uint8_t* active) {
+  return Guard([&] {
+    auto& e = engine->e;
+    const int n = e.n;
+    std::vector<uint32_t> off(static_cast<size_t>(n));
+    std::vector<int32_t> len(static_cast<size_t>(n));
+    std::vector<beam::Entry> pool(static_cast<size_t>(e.pool_cap));
+    unsigned long long top = 0;
+    HIP_CHECK(hipMemcpyAsync(&top, e.pool_top, sizeof(top), hipMemcpyDeviceToHost, e.stream));
+    e.Sync();
+    HIP_CHECK(hipMemcpyAsync(off.data(), e.row_off, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, e.stream));
+    HIP_CHECK(hipMemcpyAsync(len.data(), e.row_len, sizeof(int32_t) * n, hipMemcpyDeviceToHost, e.stream));
+    HIP_CHECK(hipMemcpyAsync(pool.data(), e.pool[e.pool_idx], sizeof(beam::Entry) * top, hipMemcpyDeviceToHost, e.stream));
+    HIP_CHECK(hipMemcpyAsync(sizes, e.size, sizeof(int32_t) * n, hipMemcpyDeviceToHost, e.stream));
+    HIP_CHECK(hipMemcpyAsync(internal, e.internal, sizeof(int32_t) * n, hipMemcpyDeviceToHost, e.stream));
+    HIP_CHECK(hipMemcpyAsync(active, e.active, n, hipMemcpyDeviceToHost, e.stream));
+    e.Sync();
+    long long pos = 0;
+    for (int r = 0; r < n; ++r) {
+      offsets[r] = pos;
+      if (!active[r]) continue;
+      std::vector<beam::Entry> row(pool.begin() + off[static_cast<size_t>(r)],
+                                   pool.begin() + off[static_cast<size_t>(r)] + len[static_cast<size_t>(r)]);
+      std::sort(row.begin(), row.end(), [](const beam::Entry& x, const beam::Entry& y) { return x.nbr < y.nbr; });
+      for (const auto& en : row) {
+        nbr[pos] = en.nbr;
+        cnt[pos] = en.cnt;
+        ++pos;
+      }
+    }
+    offsets[n] = pos;
+    return 0;
+  });
+}
+
+int beam_hip_score_pairs(beam_hip_engine* engine, const int32_t* a, const int32_t* b, int64_t count, int64_t* gain,
+                         int64_t* before) {
+  return Guard([&] {
+    auto& e = engine->e;
+    if (count <= 0) return 0;
+    int32_t* da = beam::DevAlloc<int32_t>(static_cast<size_t>(count));
+    int32_t* db = beam::DevAlloc<int32_t>(static_cast<size_t>(count));
+    long long* dg = beam::DevAlloc<long long>(static_cast<size_t>(count));
+    long long* dbf = beam::DevAlloc<long long>(static_cast<size_t>(count));
+    HIP_CHECK(hipMemcpyAsync(da, a, sizeof(int32_t) * count, hipMemcpyHostToDevice, e.stream));
+    HIP_CHECK(hipMemcpyAsync(db, b, sizeof(int32_t) * count, hipMemcpyHostToDevice, e.stream));
+    const int grid = static_cast<int>(std::min<long long>(4096, (count + 3) / 4));
+    beam::ScorePairsKernel<<<grid, 256, 0, e.stream>>>(
+        beam::QuotientView{e.pool[e.pool_idx], e.row_off, e.row_len, e.size, e.internal}, da, db, count, dg, dbf, e.err);
+    HIP_CHECK(hipMemcpyAsync(gain, dg, sizeof(long long) * count, hipMemcpyDeviceToHost, e.stream));
+    HIP_CHECK(hipMemcpyAsync(before, dbf, sizeof(long long) * count, hipMemcpyDeviceToHost, e.stream));
+    e.Sync();
+    hipFree(da);
+    hipFree(db);
+    hipFree(dg);
+    hipFree(dbf);
+    e.CheckDeviceError("score_pairs");
+    return 0;
+  });
+}
+
+int beam_hip_merge_groups(beam_hip_engine* engine, const int32_t* reps, int32_t nreps, const int32_t* starts,
+                          const int32_t* lengths, int32_t ngroups, double theta, int32_t commit, int32_t* cand_pairs,
+                          int64_t* cand_gain, int32_t* cand_count, int32_t cand_cap, int32_t* sel_pairs,
+                          int32_t* sel_count) {
+  return Guard([&] {
+    auto& e = engine->e;
+    if (ngroups <= 0) return 0;
+    if (ngroups > beam::kMaxSlotsPerBlock) throw std::invalid_argument("at most 1024 groups per call");
+    int32_t* dreps = beam::DevAlloc<int32_t>(static_cast<size_t>(nreps));
+    int32_t* dst = beam::DevAlloc<int32_t>(static_cast<size_t>(ngroups));
+    int32_t* dln = beam::DevAlloc<int32_t>(static_cast<size_t>(ngroups));
+    int32_t* dpairs = beam::DevAlloc<int32_t>(static_cast<size_t>(ngroups) * cand_cap * 2);
+    long long* dgain = beam::DevAlloc<long long>(static_cast<size_t>(ngroups) * cand_cap * 2);
+    int32_t* dcount = beam::DevAlloc<int32_t>(static_cast<size_t>(ngroups));
+    HIP_CHECK(hipMemcpyAsync(dreps, reps, sizeof(int32_t) * nreps, hipMemcpyHostToDevice, e.stream));
+    HIP_CHECK(hipMemcpyAsync(dst, starts, sizeof(int32_t) * ngroups, hipMemcpyHostToDevice, e.stream));
+    HIP_CHECK(hipMemcpyAsync(dln, lengths, sizeof(int32_t) * ngroups, hipMemcpyHostToDevice, e.stream));
+    HIP_CHECK(hipMemsetAsync(dcount, 0, sizeof(int32_t) * ngroups, e.stream));
+    beam::Counters* c = e.ctr;  // slot 0 collects counters of per-stage calls
+    e.LaunchBlock(0, ngroups, ngroups, theta, c, dreps, dst, dln, dpairs, dgain, dcount, cand_cap, commit != 0);
+    HIP_CHECK(hipMemcpyAsync(cand_count, dcount, sizeof(int32_t) * ngroups, hipMemcpyDeviceToHost, e.stream));
+    HIP_CHECK(hipMemcpyAsync(cand_pairs, dpairs, sizeof(int32_t) * ngroups * cand_cap * 2, hipMemcpyDeviceToHost, e.stream));
+    HIP_CHECK(hipMemcpyAsync(cand_gain, dgain, sizeof(long long) * ngroups * cand_cap * 2, hipMemcpyDeviceToHost, e.stream));
+    HIP_CHECK(hipMemcpyAsync(sel_count, e.sel_count, sizeof(int32_t) * ngroups, hipMemcpyDeviceToHost, e.stream));
+    HIP_CHECK(hipMemcpyAsync(sel_pairs, e.sel, sizeof(int32_t) * ngroups * beam::kMaxSelect * 2, hipMemcpyDeviceToHost, e.stream));
+    e.Sync();
+    hipFree(dreps);
+    hipFree(dst);
+    hipFree(dln);
+    hipFree(dpairs);
+    hipFree(dgain);
+    hipFree(dcount);
+    e.CheckDeviceError("merge_groups");
+    return 0;
+  });
+}
+
+int beam_hip_divide(beam_hip_engine* engine, int32_t iter, int32_t* order, int32_t* starts, int32_t* lengths,
+                    int32_t* ngroups) {
+  return Guard([&] {
+    auto& e = engine->e;
+    const int G = e.Divide(iter);
+    HIP_CHECK(hipMemcpyAsync(order, e.order, sizeof(int32_t) * e.last_active, hipMemcpyDeviceToHost, e.stream));
+    HIP_CHECK(hipMemcpyAsync(starts, e.gstart, sizeof(int32_t) * G, hipMemcpyDeviceToHost, e.stream));
+    HIP_CHECK(hipMemcpyAsync(lengths, e.glen, sizeof(int32_t) * G, hipMemcpyDeviceToHost, e.stream));
+    e.Sync();
+    *ngroups = G;
+    return 0;
+  });
+}
+
+int beam_hip_commit(beam_hip_engine* engine, const int32_t* keeps, const int32_t* removes, int32_t count) {
+  return Guard([&] {
+    auto& e = engine->e;
+    if (count <= 0) return 0;
+    const int max_pairs = beam::kMaxSlotsPerBlock * beam::kMaxSelect;
+    if (count > max_pairs) throw std::invalid_argument("too many pairs in one commit");
+    const int nslots = (count + beam::kMaxSelect - 1) / beam::kMaxSelect;
+    std::vector<int32_t> hsel(static_cast<size_t>(nslots) * beam::kMaxSelect * 2, 0);
+    std::vector<int32_t> hcount(static_cast<size_t>(nslots), 0);
+    for (int k = 0; k < count; ++k) {
+      const int s = k / beam::kMaxSelect, j = k % beam::kMaxSelect;
+      hsel[2 * (static_cast<size_t>(s) * beam::kMaxSelect + j)] = keeps[k];
+      hsel[2 * (static_cast<size_t>(s) * beam::kMaxSelect + j) + 1] = removes[k];
+      hcount[static_cast<size_t>(s)]++;
+    }
+    HIP_CHECK(hipMemcpyAsync(e.sel, hsel.data(), hsel.size() * sizeof(int32_t), hipMemcpyHostToDevice, e.stream));
+    HIP_CHECK(hipMemcpyAsync(e.sel_count, hcount.data(), hcount.size() * sizeof(int32_t), hipMemcpyHostToDevice, e.stream));
+    ++e.epoch;
+    beam::MarkCommit<<<nslots, 256, 0, e.stream>>>(e.sel, e.sel_count, nslots, e.epoch, e.mark_dest, e.dest, e.mark_keep,
+                                                   e.parent);
+    e.Commit(nslots, e.ctr);
+    e.Sync();
+    e.CheckDeviceError("commit");
+    return 0;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,582 @@
+// Per-block merge kernels for gfx950.
+//
+// One block of the blocked merge (sweg.cpp:983-1117) runs as three launches:
+//   MergeGroupsKernel  one workgroup per group of the block: legacy EA-proxy
+//                      candidates (sweg.cpp:1800-1999), exact MAGS gain of every
+//                      candidate (quotient_graph.cpp:340-462), the FP64
+//                      threshold filter (sweg.cpp:2146-2186) and the greedy
+//                      endpoint-disjoint selection (sweg.cpp:2188-2281);
+//   UnionKernel        one workgroup per selected pair: the keep row absorbs
+//                      the remove row (quotient_graph.cpp:628-742) and the
+//                      remove's neighbours are queued for re-pointing;
+//   RewriteKernel      one workgroup per queued neighbour row: entries that
+//                      point at a removed supernode now point at its keep,
+//                      coalescing with an existing keep entry.
+// Every group of a block reads the block-start state and commits happen after
+// all groups are scored, as in the reference (sweg.cpp:1097-1110, 2683-2747).
+#include "engine.hpp"
+#include "score_pair.hpp"
+
+namespace beam {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kWaves = kThreads / kWave;
+constexpr int kMaxGroup = 512;
+constexpr int kMemberSlots = 1024;
+constexpr int kSmallE = 2048;     // entries handled in LDS
+constexpr int kSmallG = 64;       // members whose proxy matrix fits in LDS
+constexpr int kSmallTC = 4096;    // target hash slots in LDS
+constexpr int kSmallC = 2016;     // 64 * 63 / 2 candidate pairs at most
+
+// LDS region shared by the candidate phase (entries, target hash, prefix
+// arrays) and, once candidates are fixed, by the scoring phase (candidate
+// arrays, per-wave join tables, then the sorted survivor list).
+constexpr int kCandPhaseBytes = kSmallE * 4 * 4 + kSmallTC * 4 + (kSmallTC + 1) * 4 * 2;
+constexpr int kCandArrayBytes = kSmallC * (4 + 4 + 8 + 8 + 4);
+constexpr int kTableBytes = kWaves * kScoreSlots * 8;
+constexpr int kRegionBytes = kCandPhaseBytes;
+static_assert(kCandArrayBytes + kTableBytes <= kRegionBytes, "scoring region must fit");
+static_assert(kSmallC * 4 <= kTableBytes, "sorted survivors reuse the join tables");
+
+struct GroupArrays {
+  int32_t* ent_t;
+  int32_t* ent_e;
+  int32_t* ent_st;
+  int32_t* ent_slot;
+  int32_t* ent_i;
+  int32_t* ent_rank;
+  int32_t* th_key;
+  uint32_t* th_cnt;     // counts, then exclusive offsets (TC + 1)
+  uint32_t* pair_pref;  // pairs per slot, then exclusive offsets (TC + 1)
+  int32_t* bucket;
+  long long* proxy;     // g x g, upper triangle
+  uint32_t* candbits;   // g x g bits
+  int32_t* cand_a;
+  int32_t* cand_b;
+  long long* cand_gain;
+  long long* cand_before;
+  int32_t* pass_idx;
+  int32_t* sorted;
+  int32_t* tables;  // per-wave join tables (always LDS)
+};
+
+// Exclusive scan in place over n entries of `data` (generic pointer), with
+// data[n] receiving the total.  All threads of the workgroup participate.
+__device__ uint32_t ScanInPlace(uint32_t* data, int n, uint32_t* wave_tot) {
+  const int per = (n + kThreads - 1) / kThreads;
+  const int begin = static_cast<int>(threadIdx.x) * per;
+  const int end = min(n, begin + per);
+  uint32_t local = 0;
+  for (int i = begin; i < end; ++i) local += data[i];
+  uint32_t total = 0;
+  uint32_t run = BlockExclusiveScan(local, wave_tot, &total);
+  for (int i = begin; i < end; ++i) {
+    const uint32_t v = data[i];
+    data[i] = run;
+    run += v;
+  }
+  if (threadIdx.x == 0) data[n] = total;
+  __syncthreads();
+  return total;
+}
+
+// Index of the last offset <= value in a non-decreasing array of n+1 offsets.
+__device__ __forceinline__ int UpperSegment(const uint32_t* off, int n, uint32_t value) {
+  int lo = 0, hi = n;  // invariant: off[lo] <= value < off[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (off[mid] <= value) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// Triangular decode of l in [0, s(s-1)/2) into 0 <= x < y < s, row-major over x.
+__device__ __forceinline__ void TriDecode(uint32_t l, uint32_t s, uint32_t* x, uint32_t* y) {
+  // Row x starts at x*s - x(x+1)/2.  Estimate with FP64 and correct.
+  const double S = static_cast<double>(s);
+  double est = (2.0 * S - 1.0 - sqrt((2.0 * S - 1.0) * (2.0 * S - 1.0) - 8.0 * static_cast<double>(l))) * 0.5;
+  int64_t xi = static_cast<int64_t>(est);
+  if (xi < 0) xi = 0;
+  if (xi > static_cast<int64_t>(s) - 2) xi = static_cast<int64_t>(s) - 2;
+  auto row_start = [&](int64_t r) { return r * static_cast<int64_t>(s) - r * (r + 1) / 2; };
+  while (xi > 0 && row_start(xi) > static_cast<int64_t>(l)) --xi;
+  while (xi + 1 <= static_cast<int64_t>(s) - 2 && row_start(xi + 1) <= static_cast<int64_t>(l)) ++xi;
+  *x = static_cast<uint32_t>(xi);
+  *y = static_cast<uint32_t>(static_cast<int64_t>(l) - row_start(xi) + xi + 1);
+}
+
+// Candidate ranking key (sweg.cpp:1926-1938): ratio desc, proxy desc,
+// rep_lo asc, rep_hi asc.
+struct RankKey {
+  double ratio;
+  long long proxy;
+  int lo, hi;
+  int valid;
+};
+
+__device__ __forceinline__ bool Better(const RankKey& l, const RankKey& r) {
+  if (l.valid != r.valid) return l.valid > r.valid;
+  if (!l.valid) return false;
+  if (l.ratio != r.ratio) return l.ratio > r.ratio;
+  if (l.proxy != r.proxy) return l.proxy > r.proxy;
+  if (l.lo != r.lo) return l.lo < r.lo;
+  return l.hi < r.hi;
+}
+
+__device__ __forceinline__ RankKey ShflKey(const RankKey& k, int mask) {
+  RankKey o;
+  o.ratio = __shfl_xor(k.ratio, mask, kWave);
+  o.proxy = __shfl_xor(k.proxy, mask, kWave);
+  o.lo = __shfl_xor(k.lo, mask, kWave);
+  o.hi = __shfl_xor(k.hi, mask, kWave);
+  o.valid = __shfl_xor(k.valid, mask, kWave);
+  return o;
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(kThreads) void MergeGroupsKernel(MergeParams p) {
+  const int slot = static_cast<int>(blockIdx.x);
+  const int gidx = p.first_group + slot;
+  const int tid = static_cast<int>(threadIdx.x);
+  if (tid == 0) p.sel_count[slot] = 0;
+  if (gidx >= p.ngroups) return;
+  const int g = p.glen[gidx];
+  if (g < 2) return;
+  if (g > kMaxGroup) {
+    if (tid == 0) atomicOr(p.err, kErrSelection);
+    return;
+  }
+  const QuotientView& q = p.q;
+
+  __shared__ int32_t m_rep[kMaxGroup], m_size[kMaxGroup], m_len[kMaxGroup], m_int[kMaxGroup];
+  __shared__ uint32_t m_off[kMaxGroup];
+  __shared__ uint32_t m_eoff[kMaxGroup + 1];
+  __shared__ long long m_aoc[kMaxGroup];
+  __shared__ int32_t mh_key[kMemberSlots];
+  __shared__ int16_t mh_val[kMemberSlots];
+  __shared__ uint32_t wave_tot[kWaves + 1];
+  __shared__ long long s_proxy[kSmallG * kSmallG];
+  __shared__ uint32_t s_candbits[kSmallG * kSmallG / 32];
+  __shared__ int16_t s_ent_i[kSmallE], s_ent_rank[kSmallE], s_bucket[kSmallE];
+  __shared__ __align__(16) unsigned char s_region[kRegionBytes];
+  __shared__ unsigned char s_used[kMaxGroup];
+  __shared__ unsigned long long s_base;
+  __shared__ int s_flag;
+  __shared__ uint32_t s_counts[4];
+
+  // Phase 1: members, member lookup table, entry offsets.
+  const int start = p.gstart[gidx];
+  for (int i = tid; i < g; i += kThreads) {
+    const int rep = p.order[start + i];
+    m_rep[i] = rep;
+    m_size[i] = q.size[rep];
+    m_len[i] = q.row_len[rep];
+    m_off[i] = q.row_off[rep];
+    m_int[i] = q.internal[rep];
+    m_aoc[i] = 0;
+    s_used[i] = 0;
+  }
+  for (int k = tid; k < kMemberSlots; k += kThreads) mh_key[k] = -1;
+  if (tid < 4) s_counts[tid] = 0;
+  __syncthreads();
+  for (int i = tid; i < g; i += kThreads) {
+    const int rep = m_rep[i];
+    uint32_t h = HashInt(static_cast<uint32_t>(rep)) & (kMemberSlots - 1);
+    while (atomicCAS(&mh_key[h], -1, rep) != -1) h = (h + 1) & (kMemberSlots - 1);
+    mh_val[h] = static_cast<int16_t>(i);
+    m_eoff[i] = static_cast<uint32_t>(m_len[i] + (m_int[i] > 0 ? 1 : 0));
+  }
+  __syncthreads();
+  const uint32_t E = ScanInPlace(m_eoff, g, wave_tot);
+
+  // Phase 2: storage -- LDS for small groups, the block scratch pool
+  // otherwise.  The per-wave join tables are always LDS.
+  uint32_t TC = 64;
+  while (TC < 2 * E) TC <<= 1;
+  const int topk = max(1, min(p.top_k, g - 1));  // sweg.cpp:1620
+  const uint32_t cmax = min(static_cast<uint32_t>(g) * static_cast<uint32_t>(topk),
+                            static_cast<uint32_t>(g) * static_cast<uint32_t>(g - 1) / 2);
+  const bool small = E <= kSmallE && g <= kSmallG;
+  GroupArrays A;
+  A.tables = reinterpret_cast<int32_t*>(s_region + kCandArrayBytes);
+  if (small) {
+    unsigned char* r = s_region;
+    A.ent_t = reinterpret_cast<int32_t*>(r); r += kSmallE * 4;
+    A.ent_e = reinterpret_cast<int32_t*>(r); r += kSmallE * 4;
+    A.ent_st = reinterpret_cast<int32_t*>(r); r += kSmallE * 4;
+    A.ent_slot = reinterpret_cast<int32_t*>(r); r += kSmallE * 4;
+    A.th_key = reinterpret_cast<int32_t*>(r); r += kSmallTC * 4;
+    A.th_cnt = reinterpret_cast<uint32_t*>(r); r += (kSmallTC + 1) * 4;
+    A.pair_pref = reinterpret_cast<uint32_t*>(r);
+    A.ent_i = nullptr;
+    A.ent_rank = nullptr;
+    A.bucket = nullptr;
+    A.proxy = s_proxy;
+    A.candbits = s_candbits;
+    // Candidate arrays alias the candidate-phase region (dead by then).
+    unsigned char* c = s_region;
+    A.cand_gain = reinterpret_cast<long long*>(c); c += kSmallC * 8;
+    A.cand_before = reinterpret_cast<long long*>(c); c += kSmallC * 8;
+    A.cand_a = reinterpret_cast<int32_t*>(c); c += kSmallC * 4;
+    A.cand_b = reinterpret_cast<int32_t*>(c); c += kSmallC * 4;
+    A.pass_idx = reinterpret_cast<int32_t*>(c);
+    A.sorted = A.tables;  // join tables are dead once scoring ends
+  } else {
+    if (tid == 0) {
+      const unsigned long long e64 = E, tc64 = TC, g64 = static_cast<unsigned long long>(g), c64 = cmax;
+      const unsigned long long bytes = e64 * 4 * 7 + tc64 * 4 + (tc64 + 1) * 8 + 16 + g64 * g64 * 8 +
+                                       (g64 * g64 / 32 + 1) * 4 + 16 + c64 * (8 + 8 + 4 + 4 + 4 + 4);
+      const unsigned long long base = atomicAdd(p.scratch_top, (bytes + 255) & ~255ULL);
+      s_base = base;
+      s_flag = base + bytes > p.scratch_cap ? 1 : 0;
+      if (s_flag) atomicOr(p.err, kErrScratch);
+    }
+    __syncthreads();
+    if (s_flag) return;
+    unsigned char* r = p.scratch + s_base;
+    auto align8 = [](unsigned char* x) {
+      return reinterpret_cast<unsigned char*>((reinterpret_cast<uintptr_t>(x) + 7) & ~uintptr_t(7));
+    };
+    A.ent_t = reinterpret_cast<int32_t*>(r); r += E * 4ULL;
+    A.ent_e = reinterpret_cast<int32_t*>(r); r += E * 4ULL;
+    A.ent_st = reinterpret_cast<int32_t*>(r); r += E * 4ULL;
+    A.ent_slot = reinterpret_cast<int32_t*>(r); r += E * 4ULL;
+    A.ent_i = reinterpret_cast<int32_t*>(r); r += E * 4ULL;
+    A.ent_rank = reinterpret_cast<int32_t*>(r); r += E * 4ULL;
+    A.bucket = reinterpret_cast<int32_t*>(r); r += E * 4ULL;
+    A.th_key = reinterpret_cast<int32_t*>(r); r += TC * 4ULL;
+    A.th_cnt = reinterpret_cast<uint32_t*>(r); r += (TC + 1) * 4ULL;
+    A.pair_pref = reinterpret_cast<uint32_t*>(r); r += (TC + 1) * 4ULL;
+    r = align8(r);
+    A.proxy = reinterpret_cast<long long*>(r); r += static_cast<unsigned long long>(g) * g * 8;
+    A.candbits = reinterpret_cast<uint32_t*>(r); r += (static_cast<unsigned long long>(g) * g / 32 + 1) * 4;
+    r = align8(r);
+    A.cand_gain = reinterpret_cast<long long*>(r); r += cmax * 8ULL;
+    A.cand_before = reinterpret_cast<long long*>(r); r += cmax * 8ULL;
+    A.cand_a = reinterpret_cast<int32_t*>(r); r += cmax * 4ULL;
+    A.cand_b = reinterpret_cast<int32_t*>(r); r += cmax * 4ULL;
+    A.pass_idx = reinterpret_cast<int32_t*>(r); r += cmax * 4ULL;
+    A.sorted = reinterpret_cast<int32_t*>(r);
+  }
+
+  // Clear the target table, the proxy matrix and the candidate bits.
+  for (uint32_t k = tid; k < TC; k += kThreads) {
+    A.th_key[k] = -1;
+    A.th_cnt[k] = 0;
+  }
+  const uint32_t gg = static_cast<uint32_t>(g) * static_cast<uint32_t>(g);
+  for (uint32_t k = tid; k < gg; k += kThreads) A.proxy[k] = 0;
+  for (uint32_t k = tid; k < (gg + 31) / 32; k += kThreads) A.candbits[k] = 0;
+  __syncthreads();
+
+  // Phase 3: entries of the row views with the doubled self entry
+  // (quotient_graph.cpp:596-616), approx_old_cost per member
+  // (sweg.cpp:1829-1850), and the target table of cross entries
+  // (buckets_by_target, sweg.cpp:1848).  Self entries never pair
+  // (sweg.cpp:1874, 1883), so they stay out of the buckets.
+  for (uint32_t e = tid; e < E; e += kThreads) {
+    const int i = UpperSegment(m_eoff, g, e);
+    const int k = static_cast<int>(e - m_eoff[i]);
+    const int rep = m_rep[i];
+    int t, cnt, st;
+    if (k < m_len[i]) {
+      const Entry en = q.pool[m_off[i] + k];
+      t = en.nbr;
+      cnt = en.cnt;
+      st = q.size[t];
+    } else {
+      t = rep;
+      cnt = 2 * m_int[i];
+      st = m_size[i];
+    }
+    A.ent_t[e] = t;
+    A.ent_e[e] = cnt;
+    A.ent_st[e] = st;
+    if (small) s_ent_i[e] = static_cast<int16_t>(i); else A.ent_i[e] = i;
+    atomicAdd(reinterpret_cast<unsigned long long*>(&m_aoc[i]),
+              static_cast<unsigned long long>(EncodeCost(rep, m_size[i], t, st, cnt)));
+    int sl = -1, rank = 0;
+    if (t != rep) {
+      uint32_t h = HashInt(static_cast<uint32_t>(t)) & (TC - 1);
+      while (true) {
+        const int prev = atomicCAS(&A.th_key[h], -1, t);
+        if (prev == -1 || prev == t) break;
+        h = (h + 1) & (TC - 1);
+      }
+      sl = static_cast<int>(h);
+      rank = static_cast<int>(atomicAdd(&A.th_cnt[h], 1U));
+    }
+    A.ent_slot[e] = sl;
+    if (small) s_ent_rank[e] = static_cast<int16_t>(rank); else A.ent_rank[e] = rank;
+  }
+  __syncthreads();
+
+  // Phase 4: bucket offsets, scatter, pairs per bucket.
+  ScanInPlace(A.th_cnt, static_cast<int>(TC), wave_tot);
+  for (uint32_t e = tid; e < E; e += kThreads) {
+    const int sl = A.ent_slot[e];
+    if (sl < 0) continue;
+    const int rank = small ? s_ent_rank[e] : A.ent_rank[e];
+    const uint32_t pos = A.th_cnt[sl] + static_cast<uint32_t>(rank);
+    if (small) s_bucket[pos] = static_cast<int16_t>(e); else A.bucket[pos] = static_cast<int32_t>(e);
+  }
+  for (uint32_t k = tid; k < TC; k += kThreads) {
+    const uint32_t s = A.th_cnt[k + 1] - A.th_cnt[k];
+    A.pair_pref[k] = s >= 2 ? s * (s - 1) / 2 : 0;
+  }
+  __syncthreads();
+  const uint32_t PT = ScanInPlace(A.pair_pref, static_cast<int>(TC), wave_tot);
+
+  // Phase 5: bucket pair expansion (sweg.cpp:1863-1903): for two members
+  // sharing target t, delta = EC(a,t,ea) + EC(b,t,eb) - EC(a+b,t,ea+eb),
+  // accumulated only when positive.
+  for (uint32_t pi = tid; pi < PT; pi += kThreads) {
+    const int sl = UpperSegment(A.pair_pref, static_cast<int>(TC), pi);
+    const uint32_t l = pi - A.pair_pref[sl];
+    const uint32_t off = A.th_cnt[sl];
+    const uint32_t s = A.th_cnt[sl + 1] - off;
+    uint32_t x, y;
+    TriDecode(l, s, &x, &y);
+    const int ex = small ? s_bucket[off + x] : A.bucket[off + x];
+    const int ey = small ? s_bucket[off + y] : A.bucket[off + y];
+    const int i = small ? s_ent_i[ex] : A.ent_i[ex];
+    const int j = small ? s_ent_i[ey] : A.ent_i[ey];
+    const int t = A.ent_t[ex];
+    const int64_t st = A.ent_st[ex];
+    const int64_t ea = A.ent_e[ex], eb = A.ent_e[ey];
+    const int64_t sa = m_size[i], sb = m_size[j];
+    const int ra = m_rep[i], rb = m_rep[j];
+    const int64_t old_cost = EncodeCost(ra, sa, t, st, ea) + EncodeCost(rb, sb, t, st, eb);
+    const int64_t new_cost = EncodeCost(ra, sa + sb, t, st, ea + eb);
+    const int64_t delta = old_cost - new_cost;
+    if (delta > 0) {
+      const int lo = min(i, j), hi = max(i, j);
+      atomicAdd(reinterpret_cast<unsigned long long*>(&A.proxy[lo * g + hi]), static_cast<unsigned long long>(delta));
+    }
+  }
+  // Phase 6: direct edges between members (sweg.cpp:1905-1924).
+  for (uint32_t e = tid; e < E; e += kThreads) {
+    if (A.ent_slot[e] < 0) continue;
+    const int t = A.ent_t[e];
+    uint32_t h = HashInt(static_cast<uint32_t>(t)) & (kMemberSlots - 1);
+    int j = -1;
+    while (true) {
+      const int key = mh_key[h];
+      if (key == t) {
+        j = mh_val[h];
+        break;
+      }
+      if (key == -1) break;
+      h = (h + 1) & (kMemberSlots - 1);
+    }
+    const int i = small ? s_ent_i[e] : A.ent_i[e];
+    if (j > i) {
+      atomicAdd(reinterpret_cast<unsigned long long*>(&A.proxy[i * g + j]),
+                static_cast<unsigned long long>(A.ent_e[e]));
+    }
+  }
+  __syncthreads();
+
+  // Phase 7: per-member top-k by (ratio desc, proxy desc, rep_lo, rep_hi)
+  // with ratio = proxy / max(1, aoc_a + aoc_b) in FP64 (sweg.cpp:1940-1977);
+  // the union of kept pairs is recorded as bits.
+  {
+    const int wave = WaveId(), lane = LaneId();
+    const int per_lane = (g + kWave - 1) / kWave;  // <= 8
+    for (int i = wave; i < g; i += kWaves) {
+      uint32_t taken = 0;
+      for (int round = 0; round < topk; ++round) {
+        RankKey best;
+        best.valid = 0;
+        best.ratio = 0.0;
+        best.proxy = 0;
+        best.lo = best.hi = 0;
+        int best_slot = -1;
+        for (int r = 0; r < per_lane; ++r) {
+          if (taken & (1U << r)) continue;
+          const int j = lane + r * kWave;
+          if (j >= g || j == i) continue;
+          const int lo = min(i, j), hi = max(i, j);
+          const long long v = A.proxy[lo * g + hi];
+          if (v <= 0) continue;
+          RankKey key;
+          key.valid = 1;
+          long long denom = m_aoc[i] + m_aoc[j];
+          if (denom < 1) denom = 1;
+          key.ratio = static_cast<double>(v) / static_cast<double>(denom);
+          key.proxy = v;
+          key.lo = min(m_rep[i], m_rep[j]);
+          key.hi = max(m_rep[i], m_rep[j]);
+          if (best_slot < 0 || Better(key, best)) {
+            best = key;
+            best_slot = r;
+          }
+        }
+        RankKey w = best;
+        int wlane = best.valid ? lane : kWave;
+#pragma unroll
+        for (int off = kWave / 2; off > 0; off >>= 1) {
+          const RankKey o = ShflKey(w, off);
+          const int olane = __shfl_xor(wlane, off, kWave);
+          if (Better(o, w)) {
+            w = o;
+            wlane = olane;
+          }
+        }
+        if (!w.valid) break;
+        if (lane == wlane) {
+          taken |= 1U << best_slot;
+          const int j = lane + best_slot * kWave;
+          const int lo = min(i, j), hi = max(i, j);
+          const uint32_t bit = static_cast<uint32_t>(lo) * static_cast<uint32_t>(g) + static_cast<uint32_t>(hi);
+          atomicOr(&A.candbits[bit >> 5], 1U << (bit & 31));
+        }
+      }
+    }
+  }
+  __syncthreads();
+
+  // Phase 8: candidate pairs in ascending (a_idx, b_idx) order
+  // (sweg.cpp:1979-1989); m_eoff now holds per-row counts.
+  for (int lo = tid; lo < g; lo += kThreads) {
+    uint32_t cnt = 0;
+    const uint32_t b0 = static_cast<uint32_t>(lo) * g + lo + 1, b1 = static_cast<uint32_t>(lo + 1) * g;
+    for (uint32_t bit = b0; bit < b1;) {
+      const uint32_t word = A.candbits[bit >> 5] >> (bit & 31);
+      const uint32_t take = min(32U - (bit & 31), b1 - bit);
+      const uint32_t m = take == 32 ? word : (word & ((1U << take) - 1U));
+      cnt += __popc(m);
+      bit += take;
+    }
+    m_eoff[lo] = cnt;
+  }
+  __syncthreads();
+  const uint32_t C = ScanInPlace(m_eoff, g, wave_tot);
+  // Bits are read into registers before the candidate arrays overwrite the
+  // region in the small layout: candbits lives outside it, so writes are safe.
+  for (int lo = tid; lo < g; lo += kThreads) {
+    uint32_t pos = m_eoff[lo];
+    for (int hi = lo + 1; hi < g; ++hi) {
+      const uint32_t bit = static_cast<uint32_t>(lo) * g + hi;
+      if ((A.candbits[bit >> 5] >> (bit & 31)) & 1U) {
+        A.cand_a[pos] = lo;
+        A.cand_b[pos] = hi;
+        ++pos;
+      }
+    }
+  }
+  __syncthreads();
+
+  // Phase 9: exact gain for every candidate, one wave per pair.
+  {
+    int32_t* tkey = A.tables + WaveId() * kScoreSlots * 2;
+    int32_t* tcnt = tkey + kScoreSlots;
+    for (uint32_t pi = static_cast<uint32_t>(WaveId()); pi < C; pi += kWaves) {
+      const int a = m_rep[A.cand_a[pi]], b = m_rep[A.cand_b[pi]];
+      long long gain = 0, before = 0;
+      ScorePairWave(q, a, b, tkey, tcnt, p.err, &gain, &before);
+      if (LaneId() == 0) {
+        A.cand_gain[pi] = gain;
+        A.cand_before[pi] = before;
+      }
+    }
+  }
+  __syncthreads();
+  if (p.dbg_count != nullptr) {
+    const long long base = static_cast<long long>(slot) * p.dbg_cap;
+    for (uint32_t pi = tid; pi < C && static_cast<int>(pi) < p.dbg_cap; pi += kThreads) {
+      p.dbg_pairs[2 * (base + pi)] = A.cand_a[pi];
+      p.dbg_pairs[2 * (base + pi) + 1] = A.cand_b[pi];
+      p.dbg_gain[2 * (base + pi)] = A.cand_gain[pi];
+      p.dbg_gain[2 * (base + pi) + 1] = A.cand_before[pi];
+    }
+    if (tid == 0) p.dbg_count[slot] = static_cast<int>(C);
+  }
+
+  // Phase 10: threshold filter (sweg.cpp:2146-2186) with the reference's FP64
+  // ratio; survivors keep their candidate order.
+  for (uint32_t pi = tid; pi < C; pi += kThreads) {
+    const long long gn = A.cand_gain[pi], bf = A.cand_before[pi];
+    if (gn > 0) atomicAdd(&s_counts[0], 1U);
+    const bool pass = gn > 0 && bf > 0 && !(static_cast<double>(gn) / static_cast<double>(bf) < p.theta);
+    A.pass_idx[pi] = pass ? 1 : 0;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    uint32_t P = 0;
+    for (uint32_t pi = 0; pi < C; ++pi) {
+      if (A.pass_idx[pi]) A.pass_idx[P++] = static_cast<int32_t>(pi);
+    }
+    s_counts[1] = P;
+  }
+  __syncthreads();
+  const uint32_t P = s_counts[1];
+
+  // Phase 11: order survivors by (gain desc, a_idx asc, b_idx asc) and pick
+  // endpoint-disjoint pairs greedily (sweg.cpp:2217-2265).  The keep is the
+  // member at the lower group position.
+  for (uint32_t x = tid; x < P; x += kThreads) {
+    const int px = A.pass_idx[x];
+    const long long gx = A.cand_gain[px];
+    const int ax = A.cand_a[px], bx = A.cand_b[px];
+    uint32_t rank = 0;
+    for (uint32_t y = 0; y < P; ++y) {
+      const int py = A.pass_idx[y];
+      const long long gy = A.cand_gain[py];
+      const int ay = A.cand_a[py], by = A.cand_b[py];
+      if (gy > gx || (gy == gx && (ay < ax || (ay == ax && by < bx)))) ++rank;
+    }
+    A.sorted[rank] = px;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int chosen = 0;
+    unsigned long long overlap = 0, work = 0;
+    long long total = 0;
+    for (uint32_t x = 0; x < P; ++x) {
+      const int px = A.sorted[x];
+      const int a = A.cand_a[px], b = A.cand_b[px];
+      if (s_used[a] || s_used[b]) {
+        ++overlap;
+        continue;
+      }
+      s_used[a] = s_used[b] = 1;
+      const int keep = m_rep[a], remove = m_rep[b];
+      p.sel[2 * (slot * kMaxSelect + chosen)] = keep;
+      p.sel[2 * (slot * kMaxSelect + chosen) + 1] = remove;
+      if (p.commit) {
+        p.mark_dest[remove] = p.epoch;
+        p.dest[remove] = keep;
+        p.mark_keep[keep] = p.epoch;
+        p.parent[remove] = keep;
+      }
+      work += static_cast<unsigned long long>(m_len[a] + m_len[b]);
+      total += A.cand_gain[px];
+      ++chosen;
+    }
+    p.sel_count[slot] = chosen;
+    Counters* c = p.ctr;
+    atomicAdd(&c->group_count, 1ULL);
+    atomicMax(&c->group_max_size, static_cast<unsigned long long>(g));
+    atomicAdd(&c->raw_pair_count, static_cast<unsigned long long>(g) * (g - 1) / 2);
+    atomicAdd(&c->candidate_pairs_after_prune, static_cast<unsigned long long>(C));
+    atomicAdd(&c->proxy_pairs_examined, static_cast<unsigned long long>(PT));
+    atomicAdd(&c->exact_gain_calls, static_cast<unsigned long long>(C));
+    atomicAdd(&c->positive_gain_count, static_cast<unsigned long long>(s_counts[0]));
+    atomicAdd(&c->above_threshold, static_cast<unsigned long long>(P));
+    atomicAdd(&c->rejected_by_threshold, static_cast<unsigned long long>(C - P));
+    atomicAdd(&c->rejected_by_overlap, overlap);
+    atomicAdd(&c->selected, static_cast<unsigned long long>(chosen));
+    atomicAdd(reinterpret_cast<unsigned long long*>(&c->total_local_gain), static_cast<unsigned long long>(total));
+    atomicAdd(&c->validation_row_entry_work, work);
+    unsigned long long entries = 0;
+    for (int i = 0; i < g; ++i) entries += static_cast<unsigned long long>(m_len[i]);
+    atomicAdd(&c->member_rows, static_cast<unsigned long long>(g));
+    atomicAdd(&c->member_entries, entries);
+  }
+}
+
+}  // namespace beam

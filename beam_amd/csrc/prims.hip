@@ -1,0 +1,226 @@
+// Device-wide primitives written for this engine: exclusive scan and a stable
+// LSD radix sort of (segment, 64-bit key, value) records, used by Divide's
+// recursive min-hash split (sweg.cpp:812-851).
+#include "prims.hpp"
+
+namespace beam {
+
+namespace {
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+__global__ __launch_bounds__(kScanThreads) void ScanTileSums(const uint32_t* in, int n, uint32_t* sums) {
+  __shared__ uint32_t wave_tot[kScanThreads / kWave + 1];
+  const int base = static_cast<int>(blockIdx.x) * kScanTile + static_cast<int>(threadIdx.x) * kScanItems;
+  uint32_t local = 0;
+  for (int k = 0; k < kScanItems; ++k) {
+    const int i = base + k;
+    if (i < n) local += in[i];
+  }
+  uint32_t total = 0;
+  BlockExclusiveScan(local, wave_tot, &total);
+  if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+// Exclusive scan of the tile sums by one workgroup, chunk by chunk.
+__global__ __launch_bounds__(1024) void ScanSums(uint32_t* sums, int count, uint32_t* total_out) {
+  __shared__ uint32_t wave_tot[1024 / kWave + 1];
+  __shared__ uint32_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int c0 = 0; c0 < count; c0 += 1024) {
+    const int i = c0 + static_cast<int>(threadIdx.x);
+    const uint32_t v = i < count ? sums[i] : 0;
+    uint32_t total = 0;
+    const uint32_t ex = BlockExclusiveScan(v, wave_tot, &total);
+    if (i < count) sums[i] = carry + ex;
+    __syncthreads();
+    if (threadIdx.x == 0) carry += total;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && total_out != nullptr) *total_out = carry;
+}
+
+__global__ __launch_bounds__(kScanThreads) void ScanApply(const uint32_t* in, int n, const uint32_t* sums,
+                                                          uint32_t* out) {
+  __shared__ uint32_t wave_tot[kScanThreads / kWave + 1];
+  const int base = static_cast<int>(blockIdx.x) * kScanTile + static_cast<int>(threadIdx.x) * kScanItems;
+  uint32_t vals[kScanItems];
+  uint32_t local = 0;
+  for (int k = 0; k < kScanItems; ++k) {
+    const int i = base + k;
+    vals[k] = i < n ? in[i] : 0;
+    local += vals[k];
+  }
+  uint32_t total = 0;
+  uint32_t run = BlockExclusiveScan(local, wave_tot, &total) + sums[blockIdx.x];
+  for (int k = 0; k < kScanItems; ++k) {
+    const int i = base + k;
+    if (i < n) out[i] = run;
+    run += vals[k];
+  }
+}
+
+constexpr int kSortThreads = 256;
+constexpr int kSortRounds = 8;
+constexpr int kSortTile = kSortThreads * kSortRounds;
+
+__device__ __forceinline__ uint32_t Digit(const SortRec& r, int pass) {
+  // Passes 0..7: bytes of the key, low to high; passes 8..11: bytes of seg.
+  return pass < 8 ? static_cast<uint32_t>((r.key >> (8 * pass)) & 0xffU)
+                  : ((r.seg >> (8 * (pass - 8))) & 0xffU);
+}
+
+__global__ __launch_bounds__(kSortThreads) void RadixHist(const SortRec* in, int n, int pass, int tiles,
+                                                          uint32_t* hist) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const int base = static_cast<int>(blockIdx.x) * kSortTile;
+  for (int r = 0; r < kSortRounds; ++r) {
+    const int i = base + r * kSortThreads + static_cast<int>(threadIdx.x);
+    if (i < n) atomicAdd(&h[Digit(in[i], pass)], 1U);
+  }
+  __syncthreads();
+  hist[threadIdx.x * tiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// Stable scatter: items are taken round by round (256 consecutive items per
+// round), ranked within each wave by ballot matching on the 8 digit bits, then
+// across waves and rounds with LDS counters.
+__global__ __launch_bounds__(kSortThreads) void RadixScatter(const SortRec* in, SortRec* out, int n, int pass,
+                                                             int tiles, const uint32_t* offsets) {
+  constexpr int kW = kSortThreads / kWave;
+  __shared__ uint32_t running[256];
+  __shared__ uint32_t wcount[kW][256];
+  const int tid = static_cast<int>(threadIdx.x);
+  const int lane = LaneId(), wave = WaveId();
+  running[tid] = offsets[tid * tiles + blockIdx.x];
+  const int base = static_cast<int>(blockIdx.x) * kSortTile;
+  const uint64_t lt_mask = (lane == 0) ? 0ULL : (~0ULL >> (64 - lane));
+  for (int r = 0; r < kSortRounds; ++r) {
+    for (int w = 0; w < kW; ++w) wcount[w][tid] = 0;
+    __syncthreads();
+    const int i = base + r * kSortThreads + tid;
+    const bool valid = i < n;
+    SortRec rec;
+    uint32_t d = 0;
+    if (valid) {
+      rec = in[i];
+      d = Digit(rec, pass);
+    }
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const bool bit = (d >> b) & 1U;
+      const uint64_t m = __ballot(bit);
+      peers &= bit ? m : ~m;
+    }
+    const uint32_t rank = static_cast<uint32_t>(__popcll(peers & lt_mask));
+    if (valid && rank == 0) wcount[wave][d] = static_cast<uint32_t>(__popcll(peers));
+    __syncthreads();
+    if (valid) {
+      uint32_t pos = running[d] + rank;
+      for (int w = 0; w < wave; ++w) pos += wcount[w][d];
+      out[pos] = rec;
+    }
+    __syncthreads();
+    uint32_t add = 0;
+    for (int w = 0; w < kW; ++w) add += wcount[w][tid];
+    running[tid] += add;
+    __syncthreads();
+  }
+}
+
+__global__ void KeyBits(const SortRec* in, int n, unsigned long long* acc) {
+  // acc[0] = OR of keys, acc[1] = AND of keys, acc[2] = OR of segs.
+  unsigned long long o = 0, a = ~0ULL, s = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    o |= in[i].key;
+    a &= in[i].key;
+    s |= in[i].seg;
+  }
+  atomicOr(&acc[0], o);
+  atomicAnd(&acc[1], a);
+  atomicOr(&acc[2], s);
+}
+
+}  // namespace
+
+void DeviceScan(const uint32_t* in, uint32_t* out, int n, uint32_t* total, ScanTemp& tmp, hipStream_t stream) {
+  const int tiles = (n + kScanTile - 1) / kScanTile;
+  if (tiles == 0) {
+    hipMemsetAsync(total, 0, sizeof(uint32_t), stream);
+    return;
+  }
+  tmp.Ensure(static_cast<size_t>(tiles));
+  ScanTileSums<<<tiles, kScanThreads, 0, stream>>>(in, n, tmp.sums);
+  ScanSums<<<1, 1024, 0, stream>>>(tmp.sums, tiles, total);
+  ScanApply<<<tiles, kScanThreads, 0, stream>>>(in, n, tmp.sums, out);
+}
+
+void ScanTemp::Ensure(size_t count) {
+  if (count <= cap) return;
+  if (sums) hipFree(sums);
+  cap = count * 2;
+  if (hipMalloc(&sums, cap * sizeof(uint32_t)) != hipSuccess) sums = nullptr;
+}
+
+ScanTemp::~ScanTemp() {
+  if (sums) hipFree(sums);
+}
+
+// Sorts recs[0..n) stably by (seg, key); returns the buffer holding the result
+// (either recs or tmp).  Digit positions that are equal across all records
+// are skipped.
+SortRec* SegmentedKeySort(SortRec* recs, SortRec* tmp, int n, SortTemp& st, hipStream_t stream) {
+  if (n <= 1) return recs;
+  const int tiles = (n + kSortTile - 1) / kSortTile;
+  st.Ensure(static_cast<size_t>(tiles) * 256);
+  unsigned long long init[3] = {0ULL, ~0ULL, 0ULL};
+  hipMemcpyAsync(st.bits, init, sizeof(init), hipMemcpyHostToDevice, stream);
+  KeyBits<<<min(1024, (n + 255) / 256), 256, 0, stream>>>(recs, n, st.bits);
+  unsigned long long bits[3];
+  hipMemcpyAsync(bits, st.bits, sizeof(bits), hipMemcpyDeviceToHost, stream);
+  hipStreamSynchronize(stream);
+  const unsigned long long varying = bits[0] & ~bits[1];
+  SortRec* src = recs;
+  SortRec* dst = tmp;
+  for (int pass = 0; pass < 12; ++pass) {
+    if (pass < 8) {
+      if (((varying >> (8 * pass)) & 0xffULL) == 0) continue;
+    } else {
+      // Segment ids start at 0, so a byte above the highest set bit is zero
+      // everywhere; a single segment skips all segment passes.
+      if ((bits[2] >> (8 * (pass - 8))) == 0) continue;
+    }
+    RadixHist<<<tiles, kSortThreads, 0, stream>>>(src, n, pass, tiles, st.hist);
+    DeviceScan(st.hist, st.hist, tiles * 256, st.total, st.scan, stream);
+    RadixScatter<<<tiles, kSortThreads, 0, stream>>>(src, dst, n, pass, tiles, st.hist);
+    SortRec* t = src;
+    src = dst;
+    dst = t;
+  }
+  return src;
+}
+
+void SortTemp::Ensure(size_t count) {
+  if (!bits) {
+    hipMalloc(&bits, 4 * sizeof(unsigned long long));
+    hipMalloc(&total, sizeof(uint32_t));
+  }
+  if (count <= cap) return;
+  if (hist) hipFree(hist);
+  cap = count * 2;
+  if (hipMalloc(&hist, cap * sizeof(uint32_t)) != hipSuccess) hist = nullptr;
+}
+
+SortTemp::~SortTemp() {
+  if (hist) hipFree(hist);
+  if (bits) hipFree(bits);
+  if (total) hipFree(total);
+}
+
+}  // namespace beam

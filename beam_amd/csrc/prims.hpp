@@ -1,0 +1,37 @@
+// Device-wide scan and stable segmented key sort (see prims.hip).
+#pragma once
+
+#include "common.hpp"
+
+namespace beam {
+
+struct SortRec {
+  unsigned long long key;  // min-hash or fallback rank
+  uint32_t seg;            // open segment index (sorted first)
+  int32_t val;             // representative
+};
+
+struct ScanTemp {
+  uint32_t* sums = nullptr;
+  size_t cap = 0;
+  void Ensure(size_t count);
+  ~ScanTemp();
+};
+
+struct SortTemp {
+  uint32_t* hist = nullptr;
+  size_t cap = 0;
+  unsigned long long* bits = nullptr;
+  uint32_t* total = nullptr;
+  ScanTemp scan;
+  void Ensure(size_t count);
+  ~SortTemp();
+};
+
+// Exclusive scan of n uint32 values (in may equal out); *total (device) gets the sum.
+void DeviceScan(const uint32_t* in, uint32_t* out, int n, uint32_t* total, ScanTemp& tmp, hipStream_t stream);
+
+// Stable sort by (seg, key); returns whichever of recs / tmp holds the result.
+SortRec* SegmentedKeySort(SortRec* recs, SortRec* tmp, int n, SortTemp& st, hipStream_t stream);
+
+}  // namespace beam

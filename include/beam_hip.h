@@ -1,0 +1,199 @@
+/*
+ * C-ABI of the MI355X (gfx950) merge-scoring backend for BEAM.
+ *
+ * This is the boundary a `--scoring-backend hip` build binds to.  It replaces
+ * the reference's link-time CUDA plugin (CMakeLists.txt:19-31;
+ * include/cuda_scoring.hpp:3-4; src/cuda_scoring.cu:670-885) with plain C
+ * entry points over host buffers: graph, state and results cross the boundary
+ * as pointers and sizes; the supernode quotient, partition and all per-block
+ * work stay resident in device memory between calls.
+ *
+ * Every entry point returns 0 on success and a negative status on failure;
+ * beam_hip_last_error() then holds the message (the reference throws
+ * std::runtime_error("<ctx> failed: <msg>"), cuda_scoring.cu:25-32, which a
+ * host wrapper reproduces from this message).  Calls on one engine must not
+ * overlap; each call is synchronous when it returns.
+ */
+#ifndef BEAM_HIP_H_
+#define BEAM_HIP_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Build mode string, "hip".  Replaces CudaScoringBuildMode()
+ * (include/cuda_scoring.hpp:3, src/cuda_scoring.cu:670). */
+const char* beam_hip_build_mode(void);
+
+/* 1 when the HIP backend is compiled in.  Replaces CudaScoringCompiled()
+ * (include/cuda_scoring.hpp:4, src/cuda_scoring.cu:672). */
+int beam_hip_compiled(void);
+
+/* Message of the last failed call on this thread. */
+const char* beam_hip_last_error(void);
+
+/* Algorithm flags of the merge path (src/main.cpp:14-56). */
+typedef struct beam_hip_options {
+  int32_t top_k;            /* --top-k, default 16 */
+  int32_t group_batch_size; /* --group-batch-size, default 64 (1..1024) */
+  int32_t divide_hash_dims; /* --divide-hash-dims, default 16 */
+  int32_t divide_max_group; /* --divide-max-group, default 512 (<= 512) */
+  int32_t ea_use_threshold; /* --ea-use-threshold (must be 1: reciprocal threshold) */
+  int32_t device;           /* HIP device ordinal */
+  uint64_t seed;            /* --seed */
+} beam_hip_options;
+
+typedef struct beam_hip_engine beam_hip_engine;
+
+/* Uploads the graph (CSR of both arcs of every edge, rows sorted, as built by
+ * LoadUndirectedEdgeList, src/graph_io/graph_io.cpp:12-106) and builds the
+ * singleton partition and its quotient (Sweg::Sweg, src/sweg.cpp:102-194;
+ * QuotientGraph::Rebuild, src/quotient_graph.cpp:80-145).  Replaces
+ * Sweg::EnsureCudaScoringCache (include/sweg.hpp:684, src/cuda_scoring.cu:676-683). */
+int beam_hip_create(const int64_t* row_ptr, const int32_t* col_idx, int32_t n, int64_t arcs,
+                    const beam_hip_options* options, beam_hip_engine** engine);
+
+/* Frees all device state.  Replaces Sweg::~Sweg in the backend TU
+ * (src/cuda_scoring.cu:674). */
+void beam_hip_destroy(beam_hip_engine* engine);
+
+/* Runs iterations 1..iterations of Divide + blocked Merge (Sweg::Run,
+ * src/sweg.cpp:651-709, with Merge's batch-ea-blocked branch,
+ * src/sweg.cpp:953-1145). */
+int beam_hip_run(beam_hip_engine* engine, int32_t iterations);
+
+/* One iteration `iter` (1-based): Divide(iter) then Merge(iter). */
+int beam_hip_iterate(beam_hip_engine* engine, int32_t iter);
+
+/* Node -> representative labels, n entries (Sweg::CurrentPartitionLabels,
+ * src/sweg.cpp:2902-2918). */
+int beam_hip_labels(beam_hip_engine* engine, int32_t* labels);
+
+/* Summary counters, BEAM_HIP_STAT_COUNT entries in the order of the
+ * BEAM_HIP_STAT_* indices (RuntimeStats, include/sweg.hpp:221-386). */
+#define BEAM_HIP_STAT_COUNT 20
+enum {
+  BEAM_HIP_STAT_DIVIDE_MAX_GROUP_SIZE = 0,
+  BEAM_HIP_STAT_DIVIDE_FALLBACK_SPLITS,
+  BEAM_HIP_STAT_MERGE_CANDIDATE_PAIRS,
+  BEAM_HIP_STAT_MERGE_SELECTED_PAIRS,
+  BEAM_HIP_STAT_MERGE_TOTAL_LOCAL_GAIN,
+  BEAM_HIP_STAT_MERGE_GROUP_COUNT,
+  BEAM_HIP_STAT_MERGE_GROUP_MAX_SIZE,
+  BEAM_HIP_STAT_MERGE_RAW_PAIR_COUNT,
+  BEAM_HIP_STAT_MERGE_CANDIDATE_PAIRS_AFTER_PRUNE,
+  BEAM_HIP_STAT_MERGE_EXACT_GAIN_CALLS,
+  BEAM_HIP_STAT_MERGE_POSITIVE_GAIN_PAIRS,
+  BEAM_HIP_STAT_MERGE_REJECTED_BY_OVERLAP,
+  BEAM_HIP_STAT_MERGE_REJECTED_BY_THRESHOLD,
+  BEAM_HIP_STAT_SELECTED_MERGES_FOR_VALIDATION,
+  BEAM_HIP_STAT_ACCEPTED_MERGES_AFTER_VALIDATION,
+  BEAM_HIP_STAT_REJECTED_NONPOSITIVE,
+  BEAM_HIP_STAT_VALIDATION_EXACT_CALLS,
+  BEAM_HIP_STAT_VALIDATION_EXACT_ROW_ENTRY_WORK,
+  BEAM_HIP_STAT_ISOLATED_GAIN_SUM,
+  BEAM_HIP_STAT_BLOCKS
+};
+int beam_hip_stats(beam_hip_engine* engine, int64_t* out);
+
+/* Per-iteration counters, 10 per recorded iteration: iteration,
+ * active_supernodes, group_count, candidate_proxy_pairs_examined,
+ * candidate_pairs_after_topk, exact_gain_calls, exact_gain_positive_count,
+ * above_threshold_count, matching_selected_count, actual_merge_count
+ * (IterationProfile, include/sweg.hpp:83-178).  Returns the number written. */
+int beam_hip_round_stats(beam_hip_engine* engine, int64_t* out, int32_t max_rounds);
+
+/* Wall times of the last beam_hip_run in ms: [0] run, [1] divide, [2] merge. */
+int beam_hip_timing(beam_hip_engine* engine, double* out);
+
+/* Re-initialises the singleton partition and its quotient from the resident
+ * graph, as constructing a new Sweg on the same graph does
+ * (src/sweg.cpp:102-194); counters and timings are cleared. */
+int beam_hip_reset(beam_hip_engine* engine);
+
+/* Enables (1) or disables (0) per-launch HIP event timing of the group
+ * kernel (candidate generation + exact scoring + selection). */
+int beam_hip_profile(beam_hip_engine* engine, int32_t enable);
+
+/* Since the last reset: out[0] = summed group-kernel time in ms (profiling
+ * on), out[1] = timed launches, out[2] = member rows read, out[3] = their
+ * cross entries, out[4] = candidate pairs scored. */
+int beam_hip_kernel_profile(beam_hip_engine* engine, double* out);
+
+/* FNV-1a 64 over node labels (Sweg::PartitionHash, src/sweg.cpp:2920-2928). */
+uint64_t beam_hip_partition_hash(const int32_t* labels, int32_t n);
+
+/* Encoding of the current partition under the MAGS-compatible objective
+ * (Sweg::Encode, src/sweg.cpp:3017-3163): out[0] = |P|, out[1] = |Cp|,
+ * out[2] = |Cm|, out[3] = loop superedges in P, out[4] = supernode count,
+ * out[5] = exact partition cost (sum of block costs). */
+int beam_hip_encoding_counts(beam_hip_engine* engine, int64_t* out);
+
+/* ---- Finer-grained boundary (per-stage calls; also used by the tests) ---- */
+
+/* Replaces the quotient and partition.  offsets has n+1 entries; row r's
+ * entries are nbr/cnt[offsets[r]..offsets[r+1]).  Node labels are reset to the
+ * given reps via `labels` (n entries). */
+int beam_hip_upload_state(beam_hip_engine* engine, const int64_t* offsets, const int32_t* nbr,
+                          const int32_t* cnt, const int32_t* sizes, const int32_t* internal,
+                          const uint8_t* active, const int32_t* labels);
+
+/* Number of cross entries over active rows. */
+int64_t beam_hip_nnz(beam_hip_engine* engine);
+
+/* Copies the quotient out with every row sorted by neighbour; nbr/cnt need
+ * beam_hip_nnz entries. */
+int beam_hip_download_state(beam_hip_engine* engine, int64_t* offsets, int32_t* nbr, int32_t* cnt,
+                            int32_t* sizes, int32_t* internal, uint8_t* active);
+
+/* Exact MAGS-compatible merge gain of count pairs on the resident quotient
+ * (QuotientGraph::ExactMergeGainPersistent, src/quotient_graph.cpp:340-470).
+ * Replaces Sweg::ScoreCandidatesCuda (include/sweg.hpp:656-658,
+ * src/cuda_scoring.cu:714-885): pairs are representatives instead of rows of a
+ * re-uploaded FlatAggCSR, results are (gain, before_cost) per pair. */
+int beam_hip_score_pairs(beam_hip_engine* engine, const int32_t* a, const int32_t* b, int64_t count,
+                         int64_t* gain, int64_t* before);
+
+/* Prepares, scores, filters and selects one block of groups (one group per
+ * entry of starts/lengths over `reps`), exactly as one block of the blocked
+ * merge (src/sweg.cpp:994-1107); with commit != 0 the selection is then
+ * committed (src/sweg.cpp:2665-2769).  Per group g: cand_count[g] candidate
+ * pairs (a_idx, b_idx) at cand_pairs[2*(g*cand_cap + k)], their (gain,
+ * before) at cand_gain[2*(g*cand_cap + k)], and sel_count[g] selected
+ * (keep, remove) at sel_pairs[2*(g*256 + k)].  ngroups <= 1024. */
+int beam_hip_merge_groups(beam_hip_engine* engine, const int32_t* reps, int32_t nreps,
+                          const int32_t* starts, const int32_t* lengths, int32_t ngroups, double theta,
+                          int32_t commit, int32_t* cand_pairs, int64_t* cand_gain, int32_t* cand_count,
+                          int32_t cand_cap, int32_t* sel_pairs, int32_t* sel_count);
+
+/* Divide for `iter` on the current partition (Sweg::Divide,
+ * src/sweg.cpp:853-890): order gets the active representatives in group
+ * order, starts/lengths the groups; *ngroups their count. */
+int beam_hip_divide(beam_hip_engine* engine, int32_t iter, int32_t* order, int32_t* starts,
+                    int32_t* lengths, int32_t* ngroups);
+
+/* Commits endpoint-disjoint (keep, remove) pairs (QuotientGraph::Merge
+ * applied in order, src/quotient_graph.cpp:628-742, plus Update_S,
+ * src/sweg.cpp:2977-3015).  count <= 262144. */
+int beam_hip_commit(beam_hip_engine* engine, const int32_t* keeps, const int32_t* removes, int32_t count);
+
+/* ---- Graph input (host side) ---- */
+
+/* Loads an undirected single-edge list with the reference's semantics and
+ * error messages (LoadUndirectedEdgeList, src/graph_io/graph_io.cpp:12-106).
+ * Returns an opaque graph or NULL (message in beam_graph_last_error). */
+void* beam_graph_load(const char* path);
+const char* beam_graph_last_error(void);
+/* n, arcs (CSRGraph::m = 2 x edges) and edge lines read (input_edges_raw). */
+void beam_graph_info(void* graph, int* n, long long* arcs, long long* input_edges_raw);
+/* Copies the CSR: row_ptr (n+1) and col (arcs), rows sorted. */
+void beam_graph_csr(void* graph, long long* row_ptr, int* col);
+void beam_graph_free(void* graph);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BEAM_HIP_H_ */

@@ -1,0 +1,150 @@
+"""Writes the golden fixtures in tests/golden/ from the compiled reference.
+
+Run in the container that holds the reference sources (not on a GPU host):
+
+    make -f oracle/ref.mk            # builds oracle/_ref/beam and gain_harness
+    python tests/golden/make_goldens.py [fa am yo]
+
+For each workload it generates the edge list with beam_amd/bin/graphgen, runs
+the reference program with the benchmark flags and `--profiling rounds`, and
+stores the `metric.*` summary and the per-iteration profile columns as JSON.
+For the fa workload it also records per-pair exact gains from the reference's
+own QuotientGraph (oracle/ref_gain_harness.cpp) along a seeded merge script.
+"""
+from __future__ import annotations
+
+import csv
+import json
+import os
+import random
+import subprocess
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+REF = os.path.join(ROOT, "oracle", "_ref", "beam")
+HARNESS = os.path.join(ROOT, "oracle", "_ref", "gain_harness")
+GEN = os.path.join(ROOT, "beam_amd", "bin", "graphgen")
+DATA = os.path.join(ROOT, "data")
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+SEEDS = {"fa": 1, "am": 2, "yo": 3, "lj": 4, "u2": 5}
+FLAGS = [
+    "--merge-mode", "batch-ea-blocked", "--scoring-backend", "cpu", "--cost-objective", "mags-compatible",
+    "--state-backend", "persistent", "--quotient-update", "incremental", "--candidate-index", "legacy",
+    "--certification", "off", "--commit-policy", "g0", "--top-k", "16", "--group-batch-size", "64",
+    "--ea-use-threshold", "--threshold-policy", "reciprocal", "--iterations", "20", "--print-offset", "0",
+    "--error-bound", "0.0",
+]
+METRICS = [
+    "n", "input_edges_raw", "graph_arcs", "partition_hash", "exact_partition_cost", "supernode_count", "P", "Cp",
+    "Cm", "num_superedges_loop", "num_superedges_nonloop", "encoding_cost_mags_x2", "cost_ratio_mags_compatible",
+    "divide_max_group_size", "divide_fallback_splits", "merge_candidate_pairs", "merge_selected_pairs",
+    "merge_total_local_gain", "merge_group_count", "merge_group_max_size", "merge_raw_pair_count",
+    "merge_candidate_pairs_after_prune", "merge_exact_gain_calls", "merge_positive_gain_pairs",
+    "merge_rejected_by_overlap", "merge_rejected_by_threshold", "isolated_gain_sum",
+    "selected_merges_for_validation", "accepted_merges_after_validation", "rejected_nonpositive",
+    "validation_exact_calls", "validation_exact_row_entry_work", "quotient_nnz_final", "runtime_run_ms",
+]
+ROUND_COLS = [
+    "iteration", "active_supernodes", "group_count", "candidate_proxy_pairs_examined",
+    "candidate_pairs_after_topk", "exact_gain_calls", "exact_gain_positive_count", "above_threshold_count",
+    "matching_selected_count", "actual_merge_count",
+]
+
+
+def graph_path(name: str) -> str:
+    os.makedirs(DATA, exist_ok=True)
+    path = os.path.join(DATA, f"{name}.txt")
+    if not os.path.exists(path):
+        subprocess.run([GEN, name, path], check=True)
+    return path
+
+
+def reference_summary(name: str) -> dict:
+    path = graph_path(name)
+    prof = os.path.join(DATA, f"{name}_rounds.csv")
+    if os.path.exists(prof):
+        os.remove(prof)
+    env = dict(os.environ, OMP_NUM_THREADS=os.environ.get("OMP_NUM_THREADS", "8"))
+    out = subprocess.run([REF, "--input", path, *FLAGS, "--seed", str(SEEDS[name]), "--profiling", "rounds",
+                          "--profile-csv", prof], check=True, capture_output=True, text=True, env=env).stdout
+    metrics = {}
+    for line in out.splitlines():
+        if line.startswith("metric."):
+            key, _, value = line[len("metric."):].partition("=")
+            if key in METRICS:
+                metrics[key] = value
+    with open(prof) as f:
+        rounds = [{c: int(r[c]) for c in ROUND_COLS} for r in csv.DictReader(f)]
+    return {"workload": name, "seed": SEEDS[name], "flags": FLAGS, "metrics": metrics, "rounds": rounds}
+
+
+def gain_vectors(name: str = "fa", merges: int = 400, queries: int = 3000, rng_seed: int = 7) -> dict:
+    """Replays seeded merges on the reference QuotientGraph and records gains.
+
+    Merges pick an adjacent pair (keep = smaller active rep of a random edge);
+    queries are adjacent pairs and random active pairs, interleaved so gains
+    are taken on several intermediate states.
+    """
+    path = graph_path(name)
+    edges = []
+    with open(path) as f:
+        for line in f:
+            u, v = line.split()[:2]
+            edges.append((int(u), int(v)))
+    n = 1 + max(max(e) for e in edges)
+    rnd = random.Random(rng_seed)
+    label = list(range(n))
+
+    def find(x):
+        while label[x] != x:
+            x = label[x]
+        return x
+
+    script = []
+    checkpoints = sorted(rnd.sample(range(merges), 6)) + [merges]
+    done = 0
+    for cp in checkpoints:
+        while done < cp:
+            u, v = edges[rnd.randrange(len(edges))]
+            a, b = find(u), find(v)
+            if a == b:
+                continue
+            keep, rem = min(a, b), max(a, b)
+            script.append(f"M {keep} {rem}")
+            label[rem] = keep
+            done += 1
+        reps = sorted({find(x) for x in range(n)})
+        for _ in range(queries // len(checkpoints)):
+            if rnd.random() < 0.6:
+                u, v = edges[rnd.randrange(len(edges))]
+                a, b = find(u), find(v)
+                if a == b:
+                    continue
+            else:
+                a, b = rnd.sample(reps, 2)
+            script.append(f"G {a} {b}")
+        for _ in range(3):
+            a, b = rnd.sample(reps, 2)
+            script.append(f"F {a} {b}")
+        script.append("C")
+    script_path = os.path.join(DATA, f"{name}_gain_script.txt")
+    with open(script_path, "w") as f:
+        f.write("\n".join(script) + "\n")
+    out = subprocess.run([HARNESS, path, script_path], check=True, capture_output=True, text=True).stdout
+    return {"workload": name, "script": script, "answers": out.splitlines()}
+
+
+def main(names: list[str]) -> None:
+    for name in names:
+        summary = reference_summary(name)
+        with open(os.path.join(OUT, f"{name}_summary.json"), "w") as f:
+            json.dump(summary, f, indent=1)
+        print(name, summary["metrics"]["partition_hash"], summary["metrics"]["runtime_run_ms"])
+    if "fa" in names:
+        with open(os.path.join(OUT, "fa_gains.json"), "w") as f:
+            json.dump(gain_vectors("fa"), f)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:] or ["fa", "am"])

@@ -1,0 +1,256 @@
+"""Parity of the HIP merge path against the CPU oracle and the reference goldens.
+
+Every check is bit-exact: integer gains/costs, candidate pair lists, selections,
+quotient rows (as multisets), Divide groups, partition hashes and counters.
+"""
+from __future__ import annotations
+
+import json
+import os
+
+import numpy as np
+import pytest
+
+import beam_amd
+from tests.oracle_lib import Oracle
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def make_engine(oracle: Oracle, seed: int, **kw) -> beam_amd.Engine:
+    row_ptr, col = oracle.csr()
+    graph = beam_amd.Graph(oracle.n, len(col), len(col) // 2, row_ptr, col)
+    return beam_amd.Engine(graph, seed=seed, **kw)
+
+
+def upload(engine: beam_amd.Engine, oracle: Oracle) -> None:
+    off, nbr, cnt, sizes, internal, active = oracle.quotient()
+    engine.upload_state(off, nbr, cnt, sizes, internal, active, oracle.labels())
+
+
+def random_merges(oracle: Oracle, rng: np.random.Generator, count: int) -> None:
+    """Applies `count` merges of adjacent supernodes (keep = smaller rep)."""
+    row_ptr, col = oracle.csr()
+    src = np.repeat(np.arange(oracle.n), np.diff(row_ptr))
+    parent = oracle.labels().astype(np.int64)
+
+    def find(x):
+        root = x
+        while parent[root] != root:
+            root = parent[root]
+        while parent[x] != root:
+            parent[x], x = root, parent[x]
+        return root
+
+    done = 0
+    while done < count:
+        k = int(rng.integers(len(col)))
+        a, b = find(int(src[k])), find(int(col[k]))
+        if a == b:
+            continue
+        keep, remove = min(a, b), max(a, b)
+        oracle.merge(keep, remove)
+        parent[remove] = keep
+        done += 1
+
+
+def assert_same_state(engine: beam_amd.Engine, oracle: Oracle) -> None:
+    e_off, e_nbr, e_cnt, e_size, e_int, e_act = engine.download_state()
+    o_off, o_nbr, o_cnt, o_size, o_int, o_act = oracle.quotient()
+    np.testing.assert_array_equal(e_act, o_act)
+    act = o_act.astype(bool)
+    np.testing.assert_array_equal(e_size[act], o_size[act])
+    np.testing.assert_array_equal(e_int[act], o_int[act])
+    np.testing.assert_array_equal(e_off, o_off)
+    np.testing.assert_array_equal(e_nbr, o_nbr)
+    np.testing.assert_array_equal(e_cnt, o_cnt)
+
+
+def test_score_pairs_match_oracle(graph_path):
+    print("case: score_pairs fa", flush=True)
+    oracle = Oracle(graph_path("fa"), seed=1)
+    engine = make_engine(oracle, seed=1)
+    rng = np.random.default_rng(11)
+    row_ptr, col = oracle.csr()
+    src = np.repeat(np.arange(oracle.n), np.diff(row_ptr))
+    for stage in range(4):
+        if stage:
+            random_merges(oracle, rng, 150)
+        upload(engine, oracle)
+        labels = oracle.labels()
+        reps = np.unique(labels)
+        ks = rng.integers(len(col), size=600)
+        pairs = {(int(labels[src[k]]), int(labels[col[k]])) for k in ks}
+        for _ in range(400):
+            a, b = rng.choice(reps, 2, replace=False)
+            pairs.add((int(a), int(b)))
+        pairs = [(a, b) for a, b in pairs if a != b]
+        a = np.array([p[0] for p in pairs], np.int32)
+        b = np.array([p[1] for p in pairs], np.int32)
+        gain, before = engine.score_pairs(a, b)
+        for i, (x, y) in enumerate(pairs):
+            assert (int(gain[i]), int(before[i])) == oracle.gain(x, y), (stage, x, y)
+
+
+def test_score_pairs_match_reference_vectors(graph_path):
+    """Replays the reference harness script and checks every recorded gain."""
+    print("case: score_pairs golden vectors", flush=True)
+    with open(os.path.join(GOLDEN, "fa_gains.json")) as f:
+        fixture = json.load(f)
+    oracle = Oracle(graph_path("fa"), seed=1)
+    engine = make_engine(oracle, seed=1)
+    answers = [line.split() for line in fixture["answers"] if line.startswith("G ")]
+    qi = 0
+    pending = []
+
+    def flush():
+        nonlocal qi
+        if not pending:
+            return
+        upload(engine, oracle)
+        a = np.array([p[0] for p in pending], np.int32)
+        b = np.array([p[1] for p in pending], np.int32)
+        gain, before = engine.score_pairs(a, b)
+        for i in range(len(pending)):
+            ans = answers[qi]
+            assert int(ans[1]) == pending[i][0] and int(ans[2]) == pending[i][1]
+            assert int(gain[i]) == int(ans[3]) and int(before[i]) == int(ans[4])
+            qi += 1
+        pending.clear()
+
+    for cmd in fixture["script"]:
+        parts = cmd.split()
+        if parts[0] == "M":
+            flush()
+            oracle.merge(int(parts[1]), int(parts[2]))
+        elif parts[0] == "G":
+            pending.append((int(parts[1]), int(parts[2])))
+    flush()
+    assert qi == len(answers)
+
+
+def _select(gains, befores, pairs, theta):
+    """Filter + greedy selection restated from sweg.cpp:2146-2265 for the check."""
+    keep = []
+    for (a, b), g, bf in zip(pairs, gains, befores):
+        if g > 0 and bf > 0 and not (float(g) / float(bf) < theta):
+            keep.append((int(g), int(a), int(b)))
+    keep.sort(key=lambda t: (-t[0], t[1], t[2]))
+    used, out = set(), []
+    for g, a, b in keep:
+        if a in used or b in used:
+            continue
+        used.update((a, b))
+        out.append((a, b))
+    return out
+
+
+@pytest.mark.parametrize("merges", [0, 300, 700])
+def test_merge_groups_match_oracle(graph_path, merges):
+    print(f"case: merge_groups fa merges={merges}", flush=True)
+    oracle = Oracle(graph_path("fa"), seed=1)
+    engine = make_engine(oracle, seed=1)
+    rng = np.random.default_rng(5 + merges)
+    random_merges(oracle, rng, merges)
+    upload(engine, oracle)
+    order, starts, lengths = oracle.divide(3)
+    theta = 1.0 / 4.0
+    multi = [i for i in range(len(starts)) if lengths[i] >= 2]
+    checked = 0
+    for b0 in range(0, len(multi), 64):
+        idx = multi[b0:b0 + 64]
+        res = engine.merge_groups(order, starts[idx], lengths[idx], theta, commit=False)
+        for j, gi in enumerate(idx):
+            q = order[starts[gi]:starts[gi] + lengths[gi]]
+            want_pairs, _ = oracle.prepare_group(q)
+            got = res[j]
+            np.testing.assert_array_equal(got["pairs"], want_pairs, err_msg=f"group {gi}")
+            for k, (a, b) in enumerate(want_pairs):
+                assert (int(got["gain"][k]), int(got["before"][k])) == oracle.gain(int(q[a]), int(q[b]))
+            sel = _select(got["gain"], got["before"], want_pairs, theta)
+            want_sel = [(int(q[a]), int(q[b])) for a, b in sel]
+            assert [tuple(map(int, s)) for s in got["selected"]] == want_sel
+            checked += 1
+        if b0 > 64 * 20:
+            break
+    assert checked > 50
+
+
+def test_commit_matches_oracle(graph_path):
+    print("case: commit fa", flush=True)
+    oracle = Oracle(graph_path("fa"), seed=1)
+    engine = make_engine(oracle, seed=1)
+    rng = np.random.default_rng(3)
+    row_ptr, col = oracle.csr()
+    src = np.repeat(np.arange(oracle.n), np.diff(row_ptr))
+    for _ in range(6):
+        labels = oracle.labels()
+        used, keeps, removes = set(), [], []
+        for k in rng.integers(len(col), size=400):
+            a, b = int(labels[src[k]]), int(labels[col[k]])
+            if a == b or a in used or b in used:
+                continue
+            used.update((a, b))
+            keeps.append(min(a, b))
+            removes.append(max(a, b))
+        engine.commit(keeps, removes)
+        assert oracle.merge_bulk(keeps, removes) == 0
+        assert_same_state(engine, oracle)
+        np.testing.assert_array_equal(engine.labels(), oracle.labels())
+
+
+@pytest.mark.parametrize("merges", [0, 1500])
+def test_divide_matches_oracle(graph_path, merges):
+    print(f"case: divide am merges={merges}", flush=True)
+    oracle = Oracle(graph_path("am"), seed=2)
+    engine = make_engine(oracle, seed=2)
+    random_merges(oracle, np.random.default_rng(9), merges)
+    upload(engine, oracle)
+    for it in (1, 2, 7):
+        o_order, o_starts, o_len = oracle.divide(it)
+        e_order, e_starts, e_len = engine.divide(it)
+        np.testing.assert_array_equal(e_starts, o_starts)
+        np.testing.assert_array_equal(e_len, o_len)
+        np.testing.assert_array_equal(e_order, o_order)
+
+
+def _golden(name):
+    with open(os.path.join(GOLDEN, f"{name}_summary.json")) as f:
+        return json.load(f)
+
+
+GOLDEN_KEYS = [
+    "merge_candidate_pairs", "merge_selected_pairs", "merge_total_local_gain", "merge_group_count",
+    "merge_group_max_size", "merge_raw_pair_count", "merge_candidate_pairs_after_prune",
+    "merge_exact_gain_calls", "merge_positive_gain_pairs", "merge_rejected_by_overlap",
+    "merge_rejected_by_threshold", "selected_merges_for_validation", "accepted_merges_after_validation",
+    "rejected_nonpositive", "validation_exact_calls", "validation_exact_row_entry_work", "isolated_gain_sum",
+    "divide_max_group_size", "divide_fallback_splits",
+]
+
+
+@pytest.mark.parametrize("name,seed", [("fa", 1), ("am", 2)])
+def test_end_to_end_matches_reference(graph_path, name, seed):
+    print(f"case: end-to-end {name}", flush=True)
+    gold = _golden(name)
+    graph = beam_amd.load_graph(graph_path(name))
+    engine = beam_amd.Engine(graph, seed=seed)
+    engine.run(20)
+    labels = engine.labels()
+    m = gold["metrics"]
+    assert beam_amd.partition_hash(labels) == int(m["partition_hash"])
+    enc = engine.encoding_counts()
+    for k in ("P", "Cp", "Cm", "supernode_count"):
+        assert enc[k] == int(m[k]), k
+    assert enc["P_loops"] == int(m["num_superedges_loop"])
+    assert enc["exact_partition_cost"] == int(m["exact_partition_cost"])
+    assert 2 * (enc["P"] + enc["Cp"] + enc["Cm"]) == int(m["encoding_cost_mags_x2"])
+    stats = engine.stats()
+    for k in GOLDEN_KEYS:
+        assert stats[k] == int(m[k]), k
+    rounds = engine.rounds()
+    assert len(rounds) == len(gold["rounds"])
+    for got, want in zip(rounds, gold["rounds"]):
+        assert got == want
