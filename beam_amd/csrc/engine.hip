@@ -29,7 +29,6 @@ thread_local std::string g_error;
 
 constexpr int kMaxIterations = 4096;
 constexpr int kCommitGrid = 512;
-constexpr int kRewriteGrid = 512;
 
 template <typename T>
 T* DevAlloc(size_t count) {
@@ -331,8 +330,12 @@ __global__ __launch_bounds__(256) void ScorePairsKernel(QuotientView q, const in
   }
 }
 
+// Marks an externally chosen batch of pairs as this block's merges and
+// queues the neighbours of every remove (the group kernel does both for the
+// pairs it selects).
 __global__ void MarkCommit(const int32_t* sel, const int32_t* sel_count, int nslots, int epoch, int32_t* mark_dest,
-                           int32_t* dest, int32_t* mark_keep, int32_t* parent) {
+                           int32_t* dest, int32_t* mark_keep, int32_t* parent, QuotientView q, int32_t* mark_row,
+                           int32_t* aff, int32_t* naff) {
   const int s = blockIdx.x;
   if (s >= nslots) return;
   for (int k = threadIdx.x; k < sel_count[s]; k += blockDim.x) {
@@ -341,6 +344,12 @@ __global__ void MarkCommit(const int32_t* sel, const int32_t* sel_count, int nsl
     dest[remove] = keep;
     mark_keep[keep] = epoch;
     parent[remove] = keep;
+    const Entry* row = q.pool + q.row_off[remove];
+    for (int e = 0; e < q.row_len[remove]; ++e) {
+      const int x = row[e].nbr;
+      if (x == keep) continue;
+      if (atomicExch(&mark_row[x], epoch) != epoch) aff[atomicAdd(naff, 1)] = x;
+    }
   }
 }
 
@@ -717,6 +726,10 @@ void Engine::LaunchBlock(int first_group, int nslots, int ngroups, double theta,
   mp.dest = dest;
   mp.mark_keep = mark_keep;
   mp.parent = parent;
+  mp.mark_row = mark_row;
+  mp.aff = aff;
+  mp.naff = &naff[epoch & 1];
+  mp.reset_top = &tops[1];
   mp.scratch = scratch_a;
   mp.scratch_cap = scratch_a_cap;
   mp.scratch_top = &tops[0];
@@ -737,7 +750,7 @@ void Engine::LaunchBlock(int first_group, int nslots, int ngroups, double theta,
     }
     HIP_CHECK(hipEventRecord(ev_pool[ev_used++], stream));
   }
-  MergeGroupsKernel<<<nslots, 256, 0, stream>>>(mp);
+  MergeGroupsKernel<<<nslots, 512, 0, stream>>>(mp);
   if (profile) HIP_CHECK(hipEventRecord(ev_pool[ev_used++], stream));
   ++blocks_run;
   if (commit) Commit(nslots, c);
@@ -759,7 +772,6 @@ void Engine::Commit(int nslots, Counters* c) {
   cp.mark_dest = mark_dest;
   cp.dest = dest;
   cp.mark_keep = mark_keep;
-  cp.mark_row = mark_row;
   cp.aff = aff;
   cp.naff = &naff[epoch & 1];
   cp.naff_next = &naff[(epoch + 1) & 1];
@@ -771,9 +783,7 @@ void Engine::Commit(int nslots, Counters* c) {
   cp.reset_top = &tops[0];
   cp.ctr = c;
   cp.err = err;
-  UnionKernel<<<kCommitGrid, 256, 0, stream>>>(cp);
-  cp.reset_top = &tops[1];
-  RewriteKernel<<<kRewriteGrid, 256, 0, stream>>>(cp);
+  CommitKernel<<<kCommitGrid, 512, 0, stream>>>(cp);
 }
 
 void Engine::EnsurePoolRoom() {
@@ -1191,8 +1201,10 @@ int beam_hip_commit(beam_hip_engine* engine, const int32_t* keeps, const int32_t
     HIP_CHECK(hipMemcpyAsync(e.sel, hsel.data(), hsel.size() * sizeof(int32_t), hipMemcpyHostToDevice, e.stream));
     HIP_CHECK(hipMemcpyAsync(e.sel_count, hcount.data(), hcount.size() * sizeof(int32_t), hipMemcpyHostToDevice, e.stream));
     ++e.epoch;
-    beam::MarkCommit<<<nslots, 256, 0, e.stream>>>(e.sel, e.sel_count, nslots, e.epoch, e.mark_dest, e.dest, e.mark_keep,
-                                                   e.parent);
+    beam::MarkCommit<<<nslots, 256, 0, e.stream>>>(
+        e.sel, e.sel_count, nslots, e.epoch, e.mark_dest, e.dest, e.mark_keep, e.parent,
+        beam::QuotientView{e.pool[e.pool_idx], e.row_off, e.row_len, e.size, e.internal}, e.mark_row, e.aff,
+        &e.naff[e.epoch & 1]);
     e.Commit(nslots, e.ctr);
     e.Sync();
     e.CheckDeviceError("commit");

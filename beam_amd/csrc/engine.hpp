@@ -50,9 +50,13 @@ struct MergeParams {
   int32_t* dest;          // keep for a tagged remove
   int32_t* mark_keep;     // epoch tag: rep is a keep of this block
   int32_t* parent;
+  int32_t* mark_row;      // epoch tag: neighbour row already queued
+  int32_t* aff;           // queued neighbour rows of the block's removes
+  int32_t* naff;
   unsigned char* scratch;
   unsigned long long scratch_cap;
   unsigned long long* scratch_top;
+  unsigned long long* reset_top;  // the commit's scratch top, zeroed here
   Counters* ctr;
   int* err;
   // Optional taps for tests: candidate pairs and (gain, before) per slot.
@@ -77,7 +81,6 @@ struct CommitParams {
   const int32_t* mark_dest;
   const int32_t* dest;
   const int32_t* mark_keep;
-  int32_t* mark_row;           // epoch tag: row already queued for rewrite
   int32_t* aff;                // queued neighbour rows
   int32_t* naff;               // queue length for this block
   int32_t* naff_next;          // queue length of the next block (zeroed here)
@@ -92,7 +95,6 @@ struct CommitParams {
 };
 
 __global__ void MergeGroupsKernel(MergeParams p);
-__global__ void UnionKernel(CommitParams p);
-__global__ void RewriteKernel(CommitParams p);
+__global__ void CommitKernel(CommitParams p);
 
 }  // namespace beam

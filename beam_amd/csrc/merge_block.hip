@@ -21,45 +21,48 @@ namespace beam {
 
 namespace {
 
-constexpr int kThreads = 256;
+constexpr int kThreads = 512;
 constexpr int kWaves = kThreads / kWave;
 constexpr int kMaxGroup = 512;
 constexpr int kMemberSlots = 1024;
-constexpr int kSmallE = 2048;     // entries handled in LDS
+constexpr int kSmallE = 2048;     // entries staged in LDS
 constexpr int kSmallG = 64;       // members whose proxy matrix fits in LDS
 constexpr int kSmallTC = 4096;    // target hash slots in LDS
 constexpr int kSmallC = 2016;     // 64 * 63 / 2 candidate pairs at most
+constexpr int kTableSlots = 256;  // per-wave join table slots (long rows only)
 
-// LDS region shared by the candidate phase (entries, target hash, prefix
-// arrays) and, once candidates are fixed, by the scoring phase (candidate
-// arrays, per-wave join tables, then the sorted survivor list).
-constexpr int kCandPhaseBytes = kSmallE * 4 * 4 + kSmallTC * 4 + (kSmallTC + 1) * 4 * 2;
-constexpr int kCandArrayBytes = kSmallC * (4 + 4 + 8 + 8 + 4);
-constexpr int kTableBytes = kWaves * kScoreSlots * 8;
-constexpr int kRegionBytes = kCandPhaseBytes;
-static_assert(kCandArrayBytes + kTableBytes <= kRegionBytes, "scoring region must fit");
-static_assert(kSmallC * 4 <= kTableBytes, "sorted survivors reuse the join tables");
+// LDS region shared by the candidate phase (target hash, buckets, prefix
+// arrays) and the scoring phase (candidate arrays, per-wave join tables,
+// survivor lists).  The staged entries live outside it for both phases.
+constexpr int kCandPhaseBytes = kSmallE * 4 + kSmallE * 2 * 2 + kSmallTC * 4 + (kSmallTC + 1) * 4 * 2;
+constexpr int kCandArrayBytes = kSmallC * (2 + 2 + 8 + 8) + (kSmallC + 1) * 4;
+constexpr int kTableBytes = kWaves * kTableSlots * 8;
+constexpr int kRegionBytes = ((kCandPhaseBytes > kCandArrayBytes + kTableBytes ? kCandPhaseBytes
+                                                                                : kCandArrayBytes + kTableBytes) + 15) & ~15;
+static_assert(2 * kSmallC * 2 <= kTableBytes, "survivor lists reuse the join tables");
 
 struct GroupArrays {
-  int32_t* ent_t;
-  int32_t* ent_e;
-  int32_t* ent_st;
-  int32_t* ent_slot;
-  int32_t* ent_i;
-  int32_t* ent_rank;
+  int32_t* ent_t;       // target rep per staged entry
+  int32_t* ent_e;       // edge count (2 * internal for the self entry)
+  int32_t* ent_st;      // target size
+  int32_t* ent_slot;    // target table slot, -1 for the self entry
   int32_t* th_key;
   uint32_t* th_cnt;     // counts, then exclusive offsets (TC + 1)
   uint32_t* pair_pref;  // pairs per slot, then exclusive offsets (TC + 1)
-  int32_t* bucket;
   long long* proxy;     // g x g, upper triangle
   uint32_t* candbits;   // g x g bits
-  int32_t* cand_a;
-  int32_t* cand_b;
+  uint16_t* cand_a;
+  uint16_t* cand_b;
   long long* cand_gain;
   long long* cand_before;
-  int32_t* pass_idx;
-  int32_t* sorted;
-  int32_t* tables;  // per-wave join tables (always LDS)
+  uint32_t* pass;       // survivor flags, then their prefix (C + 1)
+  uint16_t* plist;      // survivors in candidate order
+  uint16_t* sorted;     // survivors in selection order
+  int32_t* tables;      // per-wave join tables (always LDS)
+  // Global-mode companions of the int16 LDS arrays.
+  int32_t* g_ent_i;
+  int32_t* g_ent_rank;
+  int32_t* g_bucket;
 };
 
 // Exclusive scan in place over n entries of `data` (generic pointer), with
@@ -135,13 +138,118 @@ __device__ __forceinline__ RankKey ShflKey(const RankKey& k, int mask) {
   return o;
 }
 
+// Exact MAGS-compatible gain of members (a, b) from the staged rows, one wave
+// (quotient_graph.cpp:340-462).  The merge-join sums separate into per-row
+// streams plus a correction over common neighbours:
+//   before = sum_{x in A} BC(sa,sx,ea) + sum_{x in B} BC(sb,sx,eb) + internal terms
+//   after  = sum_{x in A} BC(sw,sx,ea) + sum_{x in B} BC(sw,sx,eb)
+//          + sum_{x in A and B} [BC(sw,sx,ea+eb) - BC(sw,sx,ea) - BC(sw,sx,eb)] + internal
+// Each staged entry carries its target's size, so no global memory is read.
+// Common neighbours are found by scanning the shorter row (broadcast reads)
+// or, for long rows, through a per-wave hash of the shorter row in chunks.
+__device__ void ScoreStaged(const GroupArrays& A, const int32_t* m_rep, const int32_t* m_size,
+                            const int32_t* m_len, const int32_t* m_int, const uint32_t* m_eoff, int a, int b,
+                            int32_t* tkey, int32_t* tcnt, int* err, long long* out_gain, long long* out_before) {
+  const int lane = LaneId();
+  const int la = m_len[a], lb = m_len[b];
+  const int s_idx = la <= lb ? a : b, l_idx = la <= lb ? b : a;
+  const int ls = m_len[s_idx], ll = m_len[l_idx];
+  const uint32_t so = m_eoff[s_idx], lo = m_eoff[l_idx];
+  const int s_rep = m_rep[s_idx], l_rep = m_rep[l_idx];
+  const int64_t ss = m_size[s_idx], sl = m_size[l_idx], sw = ss + sl;
+  int64_t before = 0, after = 0, eab_s = 0, eab_l = 0;
+  for (int k = lane; k < ls; k += kWave) {
+    const int t = A.ent_t[so + k];
+    const int64_t e = A.ent_e[so + k];
+    if (t == l_rep) {
+      eab_s = e;
+    } else {
+      const int64_t st = A.ent_st[so + k];
+      before += BlockCost(ss, st, e, false, err);
+      after += BlockCost(sw, st, e, false, err);
+    }
+  }
+  const bool direct = ls <= 48;
+  for (int k = lane; k < ll; k += kWave) {
+    const int t = A.ent_t[lo + k];
+    const int64_t e = A.ent_e[lo + k];
+    if (t == s_rep) {
+      eab_l = e;
+      continue;
+    }
+    const int64_t st = A.ent_st[lo + k];
+    before += BlockCost(sl, st, e, false, err);
+    after += BlockCost(sw, st, e, false, err);
+    if (direct) {
+      for (int j = 0; j < ls; ++j) {
+        if (A.ent_t[so + j] == t) {
+          const int64_t es = A.ent_e[so + j];
+          after += BlockCost(sw, st, es + e, false, err) - BlockCost(sw, st, es, false, err) -
+                   BlockCost(sw, st, e, false, err);
+          break;
+        }
+      }
+    }
+  }
+  if (!direct) {
+    const int chunk = kTableSlots / 2;
+    const uint32_t mask = kTableSlots - 1;
+    for (int c0 = 0; c0 < ls; c0 += chunk) {
+      const int cn = min(chunk, ls - c0);
+      for (int k = lane; k < kTableSlots; k += kWave) tkey[k] = -1;
+      WaveSync();
+      for (int k = lane; k < cn; k += kWave) {
+        const int t = A.ent_t[so + c0 + k];
+        if (t == l_rep) continue;
+        uint32_t h = HashInt(static_cast<uint32_t>(t)) & mask;
+        while (atomicCAS(&tkey[h], -1, t) != -1) h = (h + 1) & mask;
+        tcnt[h] = A.ent_e[so + c0 + k];
+      }
+      WaveSync();
+      for (int k = lane; k < ll; k += kWave) {
+        const int t = A.ent_t[lo + k];
+        if (t == s_rep) continue;
+        uint32_t h = HashInt(static_cast<uint32_t>(t)) & mask;
+        while (true) {
+          const int key = tkey[h];
+          if (key == t) {
+            const int64_t es = tcnt[h], e = A.ent_e[lo + k], st = A.ent_st[lo + k];
+            after += BlockCost(sw, st, es + e, false, err) - BlockCost(sw, st, es, false, err) -
+                     BlockCost(sw, st, e, false, err);
+            break;
+          }
+          if (key == -1) break;
+          h = (h + 1) & mask;
+        }
+      }
+      WaveSync();
+    }
+  }
+  before = WaveSum64(before);
+  after = WaveSum64(after);
+  eab_s = WaveSum64(eab_s);
+  eab_l = WaveSum64(eab_l);
+  if (lane == 0) {
+    if (eab_s != eab_l) atomicOr(err, kErrAsymmetric);
+    const int64_t sa = m_size[a], sb = m_size[b], ia = m_int[a], ib = m_int[b];
+    before += BlockCost(sa, sa, ia, true, err) + BlockCost(sb, sb, ib, true, err) +
+              BlockCost(sa, sb, eab_s, false, err);
+    after += BlockCost(sw, sw, ia + ib + eab_s, true, err);
+    *out_gain = before - after;
+    *out_before = before;
+  }
+}
+
 }  // namespace
 
 __global__ __launch_bounds__(kThreads) void MergeGroupsKernel(MergeParams p) {
   const int slot = static_cast<int>(blockIdx.x);
   const int gidx = p.first_group + slot;
   const int tid = static_cast<int>(threadIdx.x);
-  if (tid == 0) p.sel_count[slot] = 0;
+  if (tid == 0) {
+    p.sel_count[slot] = 0;
+    if (slot == 0 && p.reset_top != nullptr) *p.reset_top = 0;
+  }
   if (gidx >= p.ngroups) return;
   const int g = p.glen[gidx];
   if (g < 2) return;
@@ -149,38 +257,40 @@ __global__ __launch_bounds__(kThreads) void MergeGroupsKernel(MergeParams p) {
     if (tid == 0) atomicOr(p.err, kErrSelection);
     return;
   }
-  const QuotientView& q = p.q;
 
   __shared__ int32_t m_rep[kMaxGroup], m_size[kMaxGroup], m_len[kMaxGroup], m_int[kMaxGroup];
   __shared__ uint32_t m_off[kMaxGroup];
   __shared__ uint32_t m_eoff[kMaxGroup + 1];
+  __shared__ uint32_t m_rowcnt[kMaxGroup + 1];
   __shared__ long long m_aoc[kMaxGroup];
   __shared__ int32_t mh_key[kMemberSlots];
   __shared__ int16_t mh_val[kMemberSlots];
   __shared__ uint32_t wave_tot[kWaves + 1];
   __shared__ long long s_proxy[kSmallG * kSmallG];
   __shared__ uint32_t s_candbits[kSmallG * kSmallG / 32];
-  __shared__ int16_t s_ent_i[kSmallE], s_ent_rank[kSmallE], s_bucket[kSmallE];
+  __shared__ int32_t s_ent_t[kSmallE], s_ent_e[kSmallE], s_ent_st[kSmallE];
+  __shared__ int16_t s_ent_i[kSmallE];
   __shared__ __align__(16) unsigned char s_region[kRegionBytes];
   __shared__ unsigned char s_used[kMaxGroup];
+  __shared__ int16_t s_sel[kMaxSelect * 2];
   __shared__ unsigned long long s_base;
-  __shared__ int s_flag;
-  __shared__ uint32_t s_counts[4];
+  __shared__ int s_flag, s_chosen;
+  __shared__ uint32_t s_counts[2];
 
-  // Phase 1: members, member lookup table, entry offsets.
+  // Phase 1: members, member lookup table, staged entry offsets.
   const int start = p.gstart[gidx];
   for (int i = tid; i < g; i += kThreads) {
     const int rep = p.order[start + i];
     m_rep[i] = rep;
-    m_size[i] = q.size[rep];
-    m_len[i] = q.row_len[rep];
-    m_off[i] = q.row_off[rep];
-    m_int[i] = q.internal[rep];
+    m_size[i] = p.q.size[rep];
+    m_len[i] = p.q.row_len[rep];
+    m_off[i] = p.q.row_off[rep];
+    m_int[i] = p.q.internal[rep];
     m_aoc[i] = 0;
     s_used[i] = 0;
   }
   for (int k = tid; k < kMemberSlots; k += kThreads) mh_key[k] = -1;
-  if (tid < 4) s_counts[tid] = 0;
+  if (tid < 2) s_counts[tid] = 0;
   __syncthreads();
   for (int i = tid; i < g; i += kThreads) {
     const int rep = m_rep[i];
@@ -192,8 +302,7 @@ __global__ __launch_bounds__(kThreads) void MergeGroupsKernel(MergeParams p) {
   __syncthreads();
   const uint32_t E = ScanInPlace(m_eoff, g, wave_tot);
 
-  // Phase 2: storage -- LDS for small groups, the block scratch pool
-  // otherwise.  The per-wave join tables are always LDS.
+  // Phase 2: storage -- LDS for small groups, the block scratch pool otherwise.
   uint32_t TC = 64;
   while (TC < 2 * E) TC <<= 1;
   const int topk = max(1, min(p.top_k, g - 1));  // sweg.cpp:1620
@@ -203,32 +312,32 @@ __global__ __launch_bounds__(kThreads) void MergeGroupsKernel(MergeParams p) {
   GroupArrays A;
   A.tables = reinterpret_cast<int32_t*>(s_region + kCandArrayBytes);
   if (small) {
+    A.ent_t = s_ent_t;
+    A.ent_e = s_ent_e;
+    A.ent_st = s_ent_st;
     unsigned char* r = s_region;
-    A.ent_t = reinterpret_cast<int32_t*>(r); r += kSmallE * 4;
-    A.ent_e = reinterpret_cast<int32_t*>(r); r += kSmallE * 4;
-    A.ent_st = reinterpret_cast<int32_t*>(r); r += kSmallE * 4;
     A.ent_slot = reinterpret_cast<int32_t*>(r); r += kSmallE * 4;
+    r += kSmallE * 2 * 2;  // ent_rank and bucket (int16) follow, addressed below
     A.th_key = reinterpret_cast<int32_t*>(r); r += kSmallTC * 4;
     A.th_cnt = reinterpret_cast<uint32_t*>(r); r += (kSmallTC + 1) * 4;
     A.pair_pref = reinterpret_cast<uint32_t*>(r);
-    A.ent_i = nullptr;
-    A.ent_rank = nullptr;
-    A.bucket = nullptr;
     A.proxy = s_proxy;
     A.candbits = s_candbits;
-    // Candidate arrays alias the candidate-phase region (dead by then).
     unsigned char* c = s_region;
     A.cand_gain = reinterpret_cast<long long*>(c); c += kSmallC * 8;
     A.cand_before = reinterpret_cast<long long*>(c); c += kSmallC * 8;
-    A.cand_a = reinterpret_cast<int32_t*>(c); c += kSmallC * 4;
-    A.cand_b = reinterpret_cast<int32_t*>(c); c += kSmallC * 4;
-    A.pass_idx = reinterpret_cast<int32_t*>(c);
-    A.sorted = A.tables;  // join tables are dead once scoring ends
+    A.pass = reinterpret_cast<uint32_t*>(c); c += (kSmallC + 1) * 4;
+    A.cand_a = reinterpret_cast<uint16_t*>(c); c += kSmallC * 2;
+    A.cand_b = reinterpret_cast<uint16_t*>(c);
+    A.plist = reinterpret_cast<uint16_t*>(A.tables);
+    A.sorted = A.plist + kSmallC;
+    A.g_ent_i = A.g_ent_rank = A.g_bucket = nullptr;
   } else {
     if (tid == 0) {
       const unsigned long long e64 = E, tc64 = TC, g64 = static_cast<unsigned long long>(g), c64 = cmax;
       const unsigned long long bytes = e64 * 4 * 7 + tc64 * 4 + (tc64 + 1) * 8 + 16 + g64 * g64 * 8 +
-                                       (g64 * g64 / 32 + 1) * 4 + 16 + c64 * (8 + 8 + 4 + 4 + 4 + 4);
+                                       (g64 * g64 / 32 + 1) * 4 + 16 + c64 * (8 + 8 + 2 + 2 + 2 + 2) +
+                                       (c64 + 1) * 4 + 64;
       const unsigned long long base = atomicAdd(p.scratch_top, (bytes + 255) & ~255ULL);
       s_base = base;
       s_flag = base + bytes > p.scratch_cap ? 1 : 0;
@@ -244,9 +353,9 @@ __global__ __launch_bounds__(kThreads) void MergeGroupsKernel(MergeParams p) {
     A.ent_e = reinterpret_cast<int32_t*>(r); r += E * 4ULL;
     A.ent_st = reinterpret_cast<int32_t*>(r); r += E * 4ULL;
     A.ent_slot = reinterpret_cast<int32_t*>(r); r += E * 4ULL;
-    A.ent_i = reinterpret_cast<int32_t*>(r); r += E * 4ULL;
-    A.ent_rank = reinterpret_cast<int32_t*>(r); r += E * 4ULL;
-    A.bucket = reinterpret_cast<int32_t*>(r); r += E * 4ULL;
+    A.g_ent_i = reinterpret_cast<int32_t*>(r); r += E * 4ULL;
+    A.g_ent_rank = reinterpret_cast<int32_t*>(r); r += E * 4ULL;
+    A.g_bucket = reinterpret_cast<int32_t*>(r); r += E * 4ULL;
     A.th_key = reinterpret_cast<int32_t*>(r); r += TC * 4ULL;
     A.th_cnt = reinterpret_cast<uint32_t*>(r); r += (TC + 1) * 4ULL;
     A.pair_pref = reinterpret_cast<uint32_t*>(r); r += (TC + 1) * 4ULL;
@@ -256,11 +365,14 @@ __global__ __launch_bounds__(kThreads) void MergeGroupsKernel(MergeParams p) {
     r = align8(r);
     A.cand_gain = reinterpret_cast<long long*>(r); r += cmax * 8ULL;
     A.cand_before = reinterpret_cast<long long*>(r); r += cmax * 8ULL;
-    A.cand_a = reinterpret_cast<int32_t*>(r); r += cmax * 4ULL;
-    A.cand_b = reinterpret_cast<int32_t*>(r); r += cmax * 4ULL;
-    A.pass_idx = reinterpret_cast<int32_t*>(r); r += cmax * 4ULL;
-    A.sorted = reinterpret_cast<int32_t*>(r);
+    A.pass = reinterpret_cast<uint32_t*>(r); r += (cmax + 1) * 4ULL;
+    A.cand_a = reinterpret_cast<uint16_t*>(r); r += cmax * 2ULL;
+    A.cand_b = reinterpret_cast<uint16_t*>(r); r += cmax * 2ULL;
+    A.plist = reinterpret_cast<uint16_t*>(r); r += cmax * 2ULL;
+    A.sorted = reinterpret_cast<uint16_t*>(r);
   }
+  int16_t* s_ent_rank = reinterpret_cast<int16_t*>(s_region + kSmallE * 4);
+  int16_t* s_bucket = s_ent_rank + kSmallE;
 
   // Clear the target table, the proxy matrix and the candidate bits.
   for (uint32_t k = tid; k < TC; k += kThreads) {
@@ -272,21 +384,21 @@ __global__ __launch_bounds__(kThreads) void MergeGroupsKernel(MergeParams p) {
   for (uint32_t k = tid; k < (gg + 31) / 32; k += kThreads) A.candbits[k] = 0;
   __syncthreads();
 
-  // Phase 3: entries of the row views with the doubled self entry
-  // (quotient_graph.cpp:596-616), approx_old_cost per member
-  // (sweg.cpp:1829-1850), and the target table of cross entries
-  // (buckets_by_target, sweg.cpp:1848).  Self entries never pair
-  // (sweg.cpp:1874, 1883), so they stay out of the buckets.
+  // Phase 3: staged entries of the row views with the doubled self entry
+  // (quotient_graph.cpp:596-616) -- cross entries first, self entry last --
+  // approx_old_cost per member (sweg.cpp:1829-1850), and the target table of
+  // cross entries (buckets_by_target, sweg.cpp:1848).  Self entries never
+  // pair (sweg.cpp:1874, 1883), so they stay out of the buckets.
   for (uint32_t e = tid; e < E; e += kThreads) {
     const int i = UpperSegment(m_eoff, g, e);
     const int k = static_cast<int>(e - m_eoff[i]);
     const int rep = m_rep[i];
     int t, cnt, st;
     if (k < m_len[i]) {
-      const Entry en = q.pool[m_off[i] + k];
+      const Entry en = p.q.pool[m_off[i] + k];
       t = en.nbr;
       cnt = en.cnt;
-      st = q.size[t];
+      st = p.q.size[t];
     } else {
       t = rep;
       cnt = 2 * m_int[i];
@@ -295,7 +407,7 @@ __global__ __launch_bounds__(kThreads) void MergeGroupsKernel(MergeParams p) {
     A.ent_t[e] = t;
     A.ent_e[e] = cnt;
     A.ent_st[e] = st;
-    if (small) s_ent_i[e] = static_cast<int16_t>(i); else A.ent_i[e] = i;
+    if (small) s_ent_i[e] = static_cast<int16_t>(i); else A.g_ent_i[e] = i;
     atomicAdd(reinterpret_cast<unsigned long long*>(&m_aoc[i]),
               static_cast<unsigned long long>(EncodeCost(rep, m_size[i], t, st, cnt)));
     int sl = -1, rank = 0;
@@ -310,7 +422,7 @@ __global__ __launch_bounds__(kThreads) void MergeGroupsKernel(MergeParams p) {
       rank = static_cast<int>(atomicAdd(&A.th_cnt[h], 1U));
     }
     A.ent_slot[e] = sl;
-    if (small) s_ent_rank[e] = static_cast<int16_t>(rank); else A.ent_rank[e] = rank;
+    if (small) s_ent_rank[e] = static_cast<int16_t>(rank); else A.g_ent_rank[e] = rank;
   }
   __syncthreads();
 
@@ -319,9 +431,9 @@ __global__ __launch_bounds__(kThreads) void MergeGroupsKernel(MergeParams p) {
   for (uint32_t e = tid; e < E; e += kThreads) {
     const int sl = A.ent_slot[e];
     if (sl < 0) continue;
-    const int rank = small ? s_ent_rank[e] : A.ent_rank[e];
+    const int rank = small ? s_ent_rank[e] : A.g_ent_rank[e];
     const uint32_t pos = A.th_cnt[sl] + static_cast<uint32_t>(rank);
-    if (small) s_bucket[pos] = static_cast<int16_t>(e); else A.bucket[pos] = static_cast<int32_t>(e);
+    if (small) s_bucket[pos] = static_cast<int16_t>(e); else A.g_bucket[pos] = static_cast<int32_t>(e);
   }
   for (uint32_t k = tid; k < TC; k += kThreads) {
     const uint32_t s = A.th_cnt[k + 1] - A.th_cnt[k];
@@ -340,10 +452,10 @@ __global__ __launch_bounds__(kThreads) void MergeGroupsKernel(MergeParams p) {
     const uint32_t s = A.th_cnt[sl + 1] - off;
     uint32_t x, y;
     TriDecode(l, s, &x, &y);
-    const int ex = small ? s_bucket[off + x] : A.bucket[off + x];
-    const int ey = small ? s_bucket[off + y] : A.bucket[off + y];
-    const int i = small ? s_ent_i[ex] : A.ent_i[ex];
-    const int j = small ? s_ent_i[ey] : A.ent_i[ey];
+    const int ex = small ? s_bucket[off + x] : A.g_bucket[off + x];
+    const int ey = small ? s_bucket[off + y] : A.g_bucket[off + y];
+    const int i = small ? s_ent_i[ex] : A.g_ent_i[ex];
+    const int j = small ? s_ent_i[ey] : A.g_ent_i[ey];
     const int t = A.ent_t[ex];
     const int64_t st = A.ent_st[ex];
     const int64_t ea = A.ent_e[ex], eb = A.ent_e[ey];
@@ -372,7 +484,7 @@ __global__ __launch_bounds__(kThreads) void MergeGroupsKernel(MergeParams p) {
       if (key == -1) break;
       h = (h + 1) & (kMemberSlots - 1);
     }
-    const int i = small ? s_ent_i[e] : A.ent_i[e];
+    const int i = small ? s_ent_i[e] : A.g_ent_i[e];
     if (j > i) {
       atomicAdd(reinterpret_cast<unsigned long long*>(&A.proxy[i * g + j]),
                 static_cast<unsigned long long>(A.ent_e[e]));
@@ -440,7 +552,7 @@ __global__ __launch_bounds__(kThreads) void MergeGroupsKernel(MergeParams p) {
   __syncthreads();
 
   // Phase 8: candidate pairs in ascending (a_idx, b_idx) order
-  // (sweg.cpp:1979-1989); m_eoff now holds per-row counts.
+  // (sweg.cpp:1979-1989).
   for (int lo = tid; lo < g; lo += kThreads) {
     uint32_t cnt = 0;
     const uint32_t b0 = static_cast<uint32_t>(lo) * g + lo + 1, b1 = static_cast<uint32_t>(lo + 1) * g;
@@ -451,33 +563,31 @@ __global__ __launch_bounds__(kThreads) void MergeGroupsKernel(MergeParams p) {
       cnt += __popc(m);
       bit += take;
     }
-    m_eoff[lo] = cnt;
+    m_rowcnt[lo] = cnt;
   }
   __syncthreads();
-  const uint32_t C = ScanInPlace(m_eoff, g, wave_tot);
-  // Bits are read into registers before the candidate arrays overwrite the
-  // region in the small layout: candbits lives outside it, so writes are safe.
+  const uint32_t C = ScanInPlace(m_rowcnt, g, wave_tot);
   for (int lo = tid; lo < g; lo += kThreads) {
-    uint32_t pos = m_eoff[lo];
+    uint32_t pos = m_rowcnt[lo];
     for (int hi = lo + 1; hi < g; ++hi) {
       const uint32_t bit = static_cast<uint32_t>(lo) * g + hi;
       if ((A.candbits[bit >> 5] >> (bit & 31)) & 1U) {
-        A.cand_a[pos] = lo;
-        A.cand_b[pos] = hi;
+        A.cand_a[pos] = static_cast<uint16_t>(lo);
+        A.cand_b[pos] = static_cast<uint16_t>(hi);
         ++pos;
       }
     }
   }
   __syncthreads();
 
-  // Phase 9: exact gain for every candidate, one wave per pair.
+  // Phase 9: exact gain of every candidate from the staged rows, one wave per pair.
   {
-    int32_t* tkey = A.tables + WaveId() * kScoreSlots * 2;
-    int32_t* tcnt = tkey + kScoreSlots;
+    int32_t* tkey = A.tables + WaveId() * kTableSlots * 2;
+    int32_t* tcnt = tkey + kTableSlots;
     for (uint32_t pi = static_cast<uint32_t>(WaveId()); pi < C; pi += kWaves) {
-      const int a = m_rep[A.cand_a[pi]], b = m_rep[A.cand_b[pi]];
       long long gain = 0, before = 0;
-      ScorePairWave(q, a, b, tkey, tcnt, p.err, &gain, &before);
+      ScoreStaged(A, m_rep, m_size, m_len, m_int, m_eoff, A.cand_a[pi], A.cand_b[pi], tkey, tcnt, p.err, &gain,
+                  &before);
       if (LaneId() == 0) {
         A.cand_gain[pi] = gain;
         A.cand_before[pi] = before;
@@ -498,38 +608,36 @@ __global__ __launch_bounds__(kThreads) void MergeGroupsKernel(MergeParams p) {
 
   // Phase 10: threshold filter (sweg.cpp:2146-2186) with the reference's FP64
   // ratio; survivors keep their candidate order.
-  for (uint32_t pi = tid; pi < C; pi += kThreads) {
+  auto passes = [&](uint32_t pi) {
     const long long gn = A.cand_gain[pi], bf = A.cand_before[pi];
-    if (gn > 0) atomicAdd(&s_counts[0], 1U);
-    const bool pass = gn > 0 && bf > 0 && !(static_cast<double>(gn) / static_cast<double>(bf) < p.theta);
-    A.pass_idx[pi] = pass ? 1 : 0;
+    return gn > 0 && bf > 0 && !(static_cast<double>(gn) / static_cast<double>(bf) < p.theta);
+  };
+  for (uint32_t pi = tid; pi < C; pi += kThreads) {
+    if (A.cand_gain[pi] > 0) atomicAdd(&s_counts[0], 1U);
+    A.pass[pi] = passes(pi) ? 1U : 0U;
   }
   __syncthreads();
-  if (tid == 0) {
-    uint32_t P = 0;
-    for (uint32_t pi = 0; pi < C; ++pi) {
-      if (A.pass_idx[pi]) A.pass_idx[P++] = static_cast<int32_t>(pi);
-    }
-    s_counts[1] = P;
+  const uint32_t P = ScanInPlace(A.pass, static_cast<int>(C), wave_tot);
+  for (uint32_t pi = tid; pi < C; pi += kThreads) {
+    if (A.pass[pi + 1] != A.pass[pi]) A.plist[A.pass[pi]] = static_cast<uint16_t>(pi);
   }
   __syncthreads();
-  const uint32_t P = s_counts[1];
 
   // Phase 11: order survivors by (gain desc, a_idx asc, b_idx asc) and pick
   // endpoint-disjoint pairs greedily (sweg.cpp:2217-2265).  The keep is the
   // member at the lower group position.
   for (uint32_t x = tid; x < P; x += kThreads) {
-    const int px = A.pass_idx[x];
+    const int px = A.plist[x];
     const long long gx = A.cand_gain[px];
     const int ax = A.cand_a[px], bx = A.cand_b[px];
     uint32_t rank = 0;
     for (uint32_t y = 0; y < P; ++y) {
-      const int py = A.pass_idx[y];
+      const int py = A.plist[y];
       const long long gy = A.cand_gain[py];
       const int ay = A.cand_a[py], by = A.cand_b[py];
       if (gy > gx || (gy == gx && (ay < ax || (ay == ax && by < bx)))) ++rank;
     }
-    A.sorted[rank] = px;
+    A.sorted[rank] = static_cast<uint16_t>(px);
   }
   __syncthreads();
   if (tid == 0) {
@@ -547,6 +655,8 @@ __global__ __launch_bounds__(kThreads) void MergeGroupsKernel(MergeParams p) {
       const int keep = m_rep[a], remove = m_rep[b];
       p.sel[2 * (slot * kMaxSelect + chosen)] = keep;
       p.sel[2 * (slot * kMaxSelect + chosen) + 1] = remove;
+      s_sel[2 * chosen] = static_cast<int16_t>(a);
+      s_sel[2 * chosen + 1] = static_cast<int16_t>(b);
       if (p.commit) {
         p.mark_dest[remove] = p.epoch;
         p.dest[remove] = keep;
@@ -558,7 +668,10 @@ __global__ __launch_bounds__(kThreads) void MergeGroupsKernel(MergeParams p) {
       ++chosen;
     }
     p.sel_count[slot] = chosen;
+    s_chosen = chosen;
     Counters* c = p.ctr;
+    unsigned long long entries = 0;
+    for (int i = 0; i < g; ++i) entries += static_cast<unsigned long long>(m_len[i]);
     atomicAdd(&c->group_count, 1ULL);
     atomicMax(&c->group_max_size, static_cast<unsigned long long>(g));
     atomicAdd(&c->raw_pair_count, static_cast<unsigned long long>(g) * (g - 1) / 2);
@@ -572,10 +685,25 @@ __global__ __launch_bounds__(kThreads) void MergeGroupsKernel(MergeParams p) {
     atomicAdd(&c->selected, static_cast<unsigned long long>(chosen));
     atomicAdd(reinterpret_cast<unsigned long long*>(&c->total_local_gain), static_cast<unsigned long long>(total));
     atomicAdd(&c->validation_row_entry_work, work);
-    unsigned long long entries = 0;
-    for (int i = 0; i < g; ++i) entries += static_cast<unsigned long long>(m_len[i]);
     atomicAdd(&c->member_rows, static_cast<unsigned long long>(g));
     atomicAdd(&c->member_entries, entries);
+  }
+  __syncthreads();
+
+  // Phase 12: queue the neighbours of every remove for re-pointing; rows
+  // that turn out to be endpoints of this block are skipped by the commit.
+  if (p.commit) {
+    const int chosen = s_chosen;
+    for (int k = 0; k < chosen; ++k) {
+      const int b = s_sel[2 * k + 1];
+      const int keep = m_rep[s_sel[2 * k]];
+      const uint32_t off = m_eoff[b];
+      for (int e = tid; e < m_len[b]; e += kThreads) {
+        const int x = A.ent_t[off + e];
+        if (x == keep) continue;
+        if (atomicExch(&p.mark_row[x], p.epoch) != p.epoch) p.aff[atomicAdd(p.naff, 1)] = x;
+      }
+    }
   }
 }
 
