@@ -110,6 +110,8 @@ def library() -> ctypes.CDLL:
         "beam_hip_profile": (ctypes.c_int, [P, i32]),
         "beam_hip_kernel_profile": (ctypes.c_int, [P, P]),
         "beam_hip_partition_hash": (ctypes.c_uint64, [P, i32]),
+        "beam_hip_trace": (ctypes.c_int, [P, i32]),
+        "beam_hip_trace_fetch": (i64, [P, P, i64]),
         "beam_graph_load": (P, [ctypes.c_char_p]),
         "beam_graph_last_error": (ctypes.c_char_p, []),
         "beam_graph_info": (None, [P, P, P, P]),
@@ -132,7 +134,8 @@ def exported_symbols() -> list[str]:
         "beam_hip_round_stats", "beam_hip_timing", "beam_hip_encoding_counts", "beam_hip_upload_state",
         "beam_hip_nnz", "beam_hip_download_state", "beam_hip_score_pairs", "beam_hip_merge_groups",
         "beam_hip_divide", "beam_hip_commit", "beam_hip_reset", "beam_hip_profile",
-        "beam_hip_kernel_profile", "beam_hip_partition_hash", "beam_graph_load", "beam_graph_last_error",
+        "beam_hip_kernel_profile", "beam_hip_partition_hash", "beam_hip_trace", "beam_hip_trace_fetch",
+        "beam_graph_load", "beam_graph_last_error",
         "beam_graph_info", "beam_graph_csr", "beam_graph_free",
     ]
 
@@ -212,6 +215,16 @@ class Engine:
         _check(self.lib.beam_hip_kernel_profile(self.h, _ptr(out)))
         return {"group_kernel_ms": float(out[0]), "group_kernel_launches": int(out[1]),
                 "member_rows": int(out[2]), "member_entries": int(out[3]), "pairs_scored": int(out[4])}
+
+    def trace(self, it: int) -> None:
+        """Records the group kernel's phase stamps during iteration `it`."""
+        _check(self.lib.beam_hip_trace(self.h, it))
+
+    def trace_fetch(self) -> np.ndarray:
+        count = int(self.lib.beam_hip_trace_fetch(self.h, ctypes.c_void_p(0), 0))
+        out = np.zeros((max(count, 1), 16), dtype=np.int64)
+        self.lib.beam_hip_trace_fetch(self.h, _ptr(out), count)
+        return out[:count]
 
     def iterate(self, it: int) -> None:
         _check(self.lib.beam_hip_iterate(self.h, it))

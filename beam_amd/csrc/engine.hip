@@ -421,6 +421,7 @@ struct Engine {
   unsigned long long* enc = nullptr;
   int epoch = 0;
   long long blocks_run = 0;
+  bool tracing_now = false;
 
   // Host-side bookkeeping
   int64_t divide_max_group = 0;
@@ -433,6 +434,9 @@ struct Engine {
   // Optional per-launch timing of the group kernel (HIP events on the engine
   // stream); enabled by beam_hip_profile.
   bool profile = false;
+  int trace_iter = 0;               // iteration whose groups are traced
+  long long* trace_buf = nullptr;   // 16 values per group
+  long long trace_groups = 0;
   std::vector<hipEvent_t> ev_pool;
   size_t ev_used = 0;
   double group_kernel_ms = 0;
@@ -464,7 +468,7 @@ Engine::~Engine() {
                   open_rep, mh,       recs[0],  recs[1],   flag,     excl,       rid,       run_start, run_open,
                   run_seg,  elem_open, new_idx, gflag,     gstart,   glen,       d_total,   d_maxlen, sel,
                   sel_count, mark_dest, dest,   mark_keep, mark_row, aff,        naff,      scratch_a, scratch_u,
-                  tops,     ctr,      err,      enc};
+                  tops,     ctr,      err,      enc,        trace_buf};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (hipEvent_t ev : ev_pool) (void)hipEventDestroy(ev);
@@ -735,6 +739,7 @@ void Engine::LaunchBlock(int first_group, int nslots, int ngroups, double theta,
   mp.scratch_top = &tops[0];
   mp.ctr = c;
   mp.err = err;
+  mp.trace = (trace_buf != nullptr && tracing_now) ? trace_buf : nullptr;
   mp.dbg_pairs = dbg_pairs;
   mp.dbg_gain = dbg_gain;
   mp.dbg_count = dbg_count;
@@ -809,12 +814,20 @@ void Engine::Iterate(int iter) {
   EnsurePoolRoom();
   const double theta = 1.0 / static_cast<double>(iter + 1);  // reciprocal policy (sweg.cpp:400-406)
   Counters* c = ctr + iter;
+  tracing_now = trace_iter == iter;
+  if (tracing_now) {
+    if (trace_buf) (void)hipFree(trace_buf);
+    trace_buf = DevAlloc<long long>(16ULL * static_cast<size_t>(G) + 16);
+    HIP_CHECK(hipMemsetAsync(trace_buf, 0, sizeof(long long) * (16ULL * G + 16), stream));
+    trace_groups = G;
+  }
   const int bs = opt.group_batch_size;
   for (int first = 0; first < G; first += bs) {
     const int nslots = std::min(bs, G - first);
     LaunchBlock(first, nslots, G, theta, c, order, gstart, glen, nullptr, nullptr, nullptr, 0, true);
   }
   Sync();
+  tracing_now = false;
   const auto t2 = std::chrono::steady_clock::now();
   CheckDeviceError("merge");
   if (profile) CollectEvents();
@@ -976,6 +989,23 @@ int beam_hip_timing(beam_hip_engine* engine, double* out) {
   out[1] = engine->e.divide_ms;
   out[2] = engine->e.merge_ms;
   return 0;
+}
+
+int beam_hip_trace(beam_hip_engine* engine, int32_t iter) {
+  engine->e.trace_iter = iter;
+  return 0;
+}
+
+int64_t beam_hip_trace_fetch(beam_hip_engine* engine, int64_t* out, int64_t cap_groups) {
+  int64_t count = -1;
+  Guard([&] {
+    auto& e = engine->e;
+    count = e.trace_buf ? e.trace_groups : 0;
+    const int64_t take = std::min<int64_t>(count, cap_groups);
+    if (take > 0) HIP_CHECK(hipMemcpy(out, e.trace_buf, sizeof(long long) * 16 * take, hipMemcpyDeviceToHost));
+    return 0;
+  });
+  return count;
 }
 
 int beam_hip_reset(beam_hip_engine* engine) {

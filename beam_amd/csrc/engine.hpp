@@ -59,6 +59,9 @@ struct MergeParams {
   unsigned long long* reset_top;  // the commit's scratch top, zeroed here
   Counters* ctr;
   int* err;
+  // Optional phase trace: 16 values per group at trace[16 * group]
+  // (g, E, bucket pairs, candidates, survivors, then wall-clock stamps).
+  long long* trace;
   // Optional taps for tests: candidate pairs and (gain, before) per slot.
   int32_t* dbg_pairs;
   long long* dbg_gain;

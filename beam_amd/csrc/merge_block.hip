@@ -240,9 +240,90 @@ __device__ void ScoreStaged(const GroupArrays& A, const int32_t* m_rep, const in
   }
 }
 
+constexpr int kSeg = 16;  // lanes per pair in the small-pair scorer
+
+__device__ __forceinline__ int64_t SegSum64(int64_t v) {
+#pragma unroll
+  for (int off = kSeg / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+// True when the pair fits the 16-lane scorer: both rows at most 64 entries,
+// so each lane streams at most four entries of each row.
+__device__ __forceinline__ bool SmallPair(const int32_t* m_len, int a, int b) {
+  return m_len[a] <= 64 && m_len[b] <= 64;
+}
+
+// ScoreStaged on a 16-lane segment (four pairs per wave); common neighbours
+// are found by scanning the shorter row with broadcast reads.
+__device__ void ScoreSegment(const GroupArrays& A, const int32_t* m_rep, const int32_t* m_size,
+                             const int32_t* m_len, const int32_t* m_int, const uint32_t* m_eoff, int a, int b,
+                             bool active, int* err, long long* out_gain, long long* out_before) {
+  const int sl_lane = LaneId() & (kSeg - 1);
+  int64_t before = 0, after = 0, eab_s = 0, eab_l = 0;
+  int64_t sw = 0;
+  if (active) {
+    const int la = m_len[a], lb = m_len[b];
+    const int s_idx = la <= lb ? a : b, l_idx = la <= lb ? b : a;
+    const int ls = m_len[s_idx], ll = m_len[l_idx];
+    const uint32_t so = m_eoff[s_idx], lo = m_eoff[l_idx];
+    const int s_rep = m_rep[s_idx], l_rep = m_rep[l_idx];
+    const int64_t ss = m_size[s_idx], sl = m_size[l_idx];
+    sw = ss + sl;
+    for (int k = sl_lane; k < ls; k += kSeg) {
+      const int t = A.ent_t[so + k];
+      const int64_t e = A.ent_e[so + k];
+      if (t == l_rep) {
+        eab_s = e;
+      } else {
+        const int64_t st = A.ent_st[so + k];
+        before += BlockCost(ss, st, e, false, err);
+        after += BlockCost(sw, st, e, false, err);
+      }
+    }
+    for (int k = sl_lane; k < ll; k += kSeg) {
+      const int t = A.ent_t[lo + k];
+      const int64_t e = A.ent_e[lo + k];
+      if (t == s_rep) {
+        eab_l = e;
+        continue;
+      }
+      const int64_t st = A.ent_st[lo + k];
+      before += BlockCost(sl, st, e, false, err);
+      after += BlockCost(sw, st, e, false, err);
+      for (int j = 0; j < ls; ++j) {
+        if (A.ent_t[so + j] == t) {
+          const int64_t es = A.ent_e[so + j];
+          after += BlockCost(sw, st, es + e, false, err) - BlockCost(sw, st, es, false, err) -
+                   BlockCost(sw, st, e, false, err);
+          break;
+        }
+      }
+    }
+  }
+  before = SegSum64(before);
+  after = SegSum64(after);
+  eab_s = SegSum64(eab_s);
+  eab_l = SegSum64(eab_l);
+  if (active && sl_lane == 0) {
+    if (eab_s != eab_l) atomicOr(err, kErrAsymmetric);
+    const int64_t sa = m_size[a], sb = m_size[b], ia = m_int[a], ib = m_int[b];
+    before += BlockCost(sa, sa, ia, true, err) + BlockCost(sb, sb, ib, true, err) +
+              BlockCost(sa, sb, eab_s, false, err);
+    after += BlockCost(sw, sw, ia + ib + eab_s, true, err);
+    *out_gain = before - after;
+    *out_before = before;
+  }
+}
+
 }  // namespace
 
 __global__ __launch_bounds__(kThreads) void MergeGroupsKernel(MergeParams p) {
+  long long* trace = nullptr;
+  const long long t_start = static_cast<long long>(wall_clock64());
+  auto stamp = [&](int k) {
+    if (trace != nullptr && threadIdx.x == 0) trace[k] = static_cast<long long>(wall_clock64());
+  };
   const int slot = static_cast<int>(blockIdx.x);
   const int gidx = p.first_group + slot;
   const int tid = static_cast<int>(threadIdx.x);
@@ -256,6 +337,13 @@ __global__ __launch_bounds__(kThreads) void MergeGroupsKernel(MergeParams p) {
   if (g > kMaxGroup) {
     if (tid == 0) atomicOr(p.err, kErrSelection);
     return;
+  }
+  if (p.trace != nullptr) {
+    trace = p.trace + 16LL * gidx;
+    if (tid == 0) {
+      trace[0] = g;
+      trace[5] = t_start;
+    }
   }
 
   __shared__ int32_t m_rep[kMaxGroup], m_size[kMaxGroup], m_len[kMaxGroup], m_int[kMaxGroup];
@@ -301,6 +389,7 @@ __global__ __launch_bounds__(kThreads) void MergeGroupsKernel(MergeParams p) {
   }
   __syncthreads();
   const uint32_t E = ScanInPlace(m_eoff, g, wave_tot);
+  stamp(6);
 
   // Phase 2: storage -- LDS for small groups, the block scratch pool otherwise.
   uint32_t TC = 64;
@@ -426,6 +515,7 @@ __global__ __launch_bounds__(kThreads) void MergeGroupsKernel(MergeParams p) {
   }
   __syncthreads();
 
+  stamp(7);
   // Phase 4: bucket offsets, scatter, pairs per bucket.
   ScanInPlace(A.th_cnt, static_cast<int>(TC), wave_tot);
   for (uint32_t e = tid; e < E; e += kThreads) {
@@ -492,58 +582,90 @@ __global__ __launch_bounds__(kThreads) void MergeGroupsKernel(MergeParams p) {
   }
   __syncthreads();
 
+  stamp(8);
   // Phase 7: per-member top-k by (ratio desc, proxy desc, rep_lo, rep_hi)
   // with ratio = proxy / max(1, aoc_a + aoc_b) in FP64 (sweg.cpp:1940-1977);
-  // the union of kept pairs is recorded as bits.
+  // the union of kept pairs is recorded as bits.  Within member i's row,
+  // (rep_lo, rep_hi) = (min, max)(rep_i, rep_j) orders exactly as rep_j
+  // ascending, so the key is (ratio desc, proxy desc, rep_j asc).  A row
+  // with at most k candidates keeps them all without ranking.
   {
     const int wave = WaveId(), lane = LaneId();
     const int per_lane = (g + kWave - 1) / kWave;  // <= 8
     for (int i = wave; i < g; i += kWaves) {
-      uint32_t taken = 0;
-      for (int round = 0; round < topk; ++round) {
-        RankKey best;
-        best.valid = 0;
-        best.ratio = 0.0;
-        best.proxy = 0;
-        best.lo = best.hi = 0;
-        int best_slot = -1;
+      int cnt = 0;
+      for (int r = 0; r < per_lane; ++r) {
+        const int j = lane + r * kWave;
+        if (j < g && j != i && A.proxy[min(i, j) * g + max(i, j)] > 0) ++cnt;
+      }
+      cnt = WaveSum32(cnt);
+      if (cnt <= topk) {
         for (int r = 0; r < per_lane; ++r) {
-          if (taken & (1U << r)) continue;
           const int j = lane + r * kWave;
-          if (j >= g || j == i) continue;
-          const int lo = min(i, j), hi = max(i, j);
-          const long long v = A.proxy[lo * g + hi];
-          if (v <= 0) continue;
-          RankKey key;
-          key.valid = 1;
-          long long denom = m_aoc[i] + m_aoc[j];
-          if (denom < 1) denom = 1;
-          key.ratio = static_cast<double>(v) / static_cast<double>(denom);
-          key.proxy = v;
-          key.lo = min(m_rep[i], m_rep[j]);
-          key.hi = max(m_rep[i], m_rep[j]);
-          if (best_slot < 0 || Better(key, best)) {
-            best = key;
-            best_slot = r;
+          if (j < g && j != i && A.proxy[min(i, j) * g + max(i, j)] > 0) {
+            const uint32_t bit = static_cast<uint32_t>(min(i, j)) * g + static_cast<uint32_t>(max(i, j));
+            atomicOr(&A.candbits[bit >> 5], 1U << (bit & 31));
           }
         }
-        RankKey w = best;
-        int wlane = best.valid ? lane : kWave;
+        continue;
+      }
+      double ratio[8];
+      long long pv[8];
+      uint32_t live = 0;
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        ratio[r] = 0.0;
+        pv[r] = 0;
+        const int j = lane + r * kWave;
+        if (r < per_lane && j < g && j != i) {
+          const long long v = A.proxy[min(i, j) * g + max(i, j)];
+          if (v > 0) {
+            long long denom = m_aoc[i] + m_aoc[j];
+            if (denom < 1) denom = 1;
+            ratio[r] = static_cast<double>(v) / static_cast<double>(denom);
+            pv[r] = v;
+            live |= 1U << r;
+          }
+        }
+      }
+      for (int round = 0; round < topk; ++round) {
+        double br = 0.0;
+        long long bp = 0;
+        int brep = 0x7fffffff, bslot = -1;
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+          if (!((live >> r) & 1U)) continue;
+          const int rep = m_rep[lane + r * kWave];
+          if (bslot < 0 || ratio[r] > br || (ratio[r] == br && (pv[r] > bp || (pv[r] == bp && rep < brep)))) {
+            br = ratio[r];
+            bp = pv[r];
+            brep = rep;
+            bslot = r;
+          }
+        }
+        int valid = bslot >= 0 ? 1 : 0;
+        int wlane = lane;
 #pragma unroll
         for (int off = kWave / 2; off > 0; off >>= 1) {
-          const RankKey o = ShflKey(w, off);
+          const double orat = __shfl_xor(br, off, kWave);
+          const long long opv = __shfl_xor(bp, off, kWave);
+          const int orep = __shfl_xor(brep, off, kWave);
+          const int ovalid = __shfl_xor(valid, off, kWave);
           const int olane = __shfl_xor(wlane, off, kWave);
-          if (Better(o, w)) {
-            w = o;
+          const bool take = ovalid && (!valid || orat > br || (orat == br && (opv > bp || (opv == bp && orep < brep))));
+          if (take) {
+            br = orat;
+            bp = opv;
+            brep = orep;
+            valid = ovalid;
             wlane = olane;
           }
         }
-        if (!w.valid) break;
+        if (!valid) break;
         if (lane == wlane) {
-          taken |= 1U << best_slot;
-          const int j = lane + best_slot * kWave;
-          const int lo = min(i, j), hi = max(i, j);
-          const uint32_t bit = static_cast<uint32_t>(lo) * static_cast<uint32_t>(g) + static_cast<uint32_t>(hi);
+          live &= ~(1U << bslot);
+          const int j = lane + bslot * kWave;
+          const uint32_t bit = static_cast<uint32_t>(min(i, j)) * g + static_cast<uint32_t>(max(i, j));
           atomicOr(&A.candbits[bit >> 5], 1U << (bit & 31));
         }
       }
@@ -551,6 +673,7 @@ __global__ __launch_bounds__(kThreads) void MergeGroupsKernel(MergeParams p) {
   }
   __syncthreads();
 
+  stamp(9);
   // Phase 8: candidate pairs in ascending (a_idx, b_idx) order
   // (sweg.cpp:1979-1989).
   for (int lo = tid; lo < g; lo += kThreads) {
@@ -580,14 +703,30 @@ __global__ __launch_bounds__(kThreads) void MergeGroupsKernel(MergeParams p) {
   }
   __syncthreads();
 
-  // Phase 9: exact gain of every candidate from the staged rows, one wave per pair.
+  stamp(10);
+  // Phase 9: exact gain of every candidate from the staged rows: pairs of
+  // short rows on 16-lane segments (four per wave), the rest one wave each.
   {
+    const int seg = LaneId() / kSeg;
+    for (uint32_t base = static_cast<uint32_t>(WaveId()) * 4; base < C; base += kWaves * 4) {
+      const uint32_t pi = base + static_cast<uint32_t>(seg);
+      const bool in_range = pi < C;
+      const int a = in_range ? A.cand_a[pi] : 0, b = in_range ? A.cand_b[pi] : 0;
+      const bool mine = in_range && SmallPair(m_len, a, b);
+      long long gain = 0, before = 0;
+      ScoreSegment(A, m_rep, m_size, m_len, m_int, m_eoff, a, b, mine, p.err, &gain, &before);
+      if (mine && (LaneId() & (kSeg - 1)) == 0) {
+        A.cand_gain[pi] = gain;
+        A.cand_before[pi] = before;
+      }
+    }
     int32_t* tkey = A.tables + WaveId() * kTableSlots * 2;
     int32_t* tcnt = tkey + kTableSlots;
     for (uint32_t pi = static_cast<uint32_t>(WaveId()); pi < C; pi += kWaves) {
+      const int a = A.cand_a[pi], b = A.cand_b[pi];
+      if (SmallPair(m_len, a, b)) continue;
       long long gain = 0, before = 0;
-      ScoreStaged(A, m_rep, m_size, m_len, m_int, m_eoff, A.cand_a[pi], A.cand_b[pi], tkey, tcnt, p.err, &gain,
-                  &before);
+      ScoreStaged(A, m_rep, m_size, m_len, m_int, m_eoff, a, b, tkey, tcnt, p.err, &gain, &before);
       if (LaneId() == 0) {
         A.cand_gain[pi] = gain;
         A.cand_before[pi] = before;
@@ -595,6 +734,7 @@ __global__ __launch_bounds__(kThreads) void MergeGroupsKernel(MergeParams p) {
     }
   }
   __syncthreads();
+  stamp(11);
   if (p.dbg_count != nullptr) {
     const long long base = static_cast<long long>(slot) * p.dbg_cap;
     for (uint32_t pi = tid; pi < C && static_cast<int>(pi) < p.dbg_cap; pi += kThreads) {
@@ -690,6 +830,13 @@ __global__ __launch_bounds__(kThreads) void MergeGroupsKernel(MergeParams p) {
   }
   __syncthreads();
 
+  stamp(12);
+  if (trace != nullptr && tid == 0) {
+    trace[1] = E;
+    trace[2] = PT;
+    trace[3] = C;
+    trace[4] = P;
+  }
   // Phase 12: queue the neighbours of every remove for re-pointing; rows
   // that turn out to be endpoints of this block are skipped by the commit.
   if (p.commit) {
@@ -705,6 +852,8 @@ __global__ __launch_bounds__(kThreads) void MergeGroupsKernel(MergeParams p) {
       }
     }
   }
+  __syncthreads();
+  stamp(13);
 }
 
 }  // namespace beam
