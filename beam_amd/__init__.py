@@ -36,6 +36,7 @@ STAT_NAMES = [
     "validation_exact_row_entry_work",
     "isolated_gain_sum",
     "blocks",
+    "windows",
 ]
 
 ROUND_NAMES = [

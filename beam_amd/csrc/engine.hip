@@ -29,6 +29,7 @@ thread_local std::string g_error;
 
 constexpr int kMaxIterations = 4096;
 constexpr int kCommitGrid = 512;
+constexpr int kResolveGrid = 256;
 
 template <typename T>
 T* DevAlloc(size_t count) {
@@ -405,6 +406,12 @@ struct Engine {
   // Merge
   int32_t* sel = nullptr;
   int32_t* sel_count = nullptr;
+  long long* slot_stats = nullptr;
+  unsigned long long* touch = nullptr;
+  int32_t* commit_out = nullptr;  // [0] committed slots, [1] committed blocks
+  unsigned int* done = nullptr;
+  int window_blocks = 32;
+  long long windows_run = 0;
   int32_t* mark_dest = nullptr;
   int32_t* dest = nullptr;
   int32_t* mark_keep = nullptr;
@@ -419,7 +426,7 @@ struct Engine {
   Counters* ctr = nullptr;             // per iteration
   int* err = nullptr;
   unsigned long long* enc = nullptr;
-  int epoch = 0;
+  long long epoch = 0;
   long long blocks_run = 0;
   bool tracing_now = false;
 
@@ -449,10 +456,10 @@ struct Engine {
   void CheckDeviceError(const char* where);
   int Divide(int iter);
   void MergeBlocks(int iter, int ngroups, double theta, Counters* c);
-  void LaunchBlock(int first_group, int nslots, int ngroups, double theta, Counters* c, const int32_t* order_arr,
-                   const int32_t* gstart_arr, const int32_t* glen_arr, int32_t* dbg_pairs, long long* dbg_gain,
-                   int32_t* dbg_count, int dbg_cap, bool commit);
-  void Commit(int nslots, Counters* c);
+  int LaunchWindow(int first_group, int nslots, int ngroups, int block_size, double theta, Counters* c,
+                   const int32_t* order_arr, const int32_t* gstart_arr, const int32_t* glen_arr, int32_t* dbg_pairs,
+                   long long* dbg_gain, int32_t* dbg_count, int dbg_cap, bool commit);
+  void Commit(Counters* c);
   void EnsurePoolRoom();
   void Iterate(int iter);
   void Reset();
@@ -468,6 +475,7 @@ Engine::~Engine() {
                   open_rep, mh,       recs[0],  recs[1],   flag,     excl,       rid,       run_start, run_open,
                   run_seg,  elem_open, new_idx, gflag,     gstart,   glen,       d_total,   d_maxlen, sel,
                   sel_count, mark_dest, dest,   mark_keep, mark_row, aff,        naff,      scratch_a, scratch_u,
+                  slot_stats, touch,  commit_out, done,
                   tops,     ctr,      err,      enc,        trace_buf};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -536,8 +544,12 @@ void Engine::Create(const int64_t* h_row_ptr, const int32_t* h_col, int n_, long
   d_total = DevAlloc<uint32_t>(4);
   d_maxlen = DevAlloc<unsigned long long>(1);
 
-  sel = DevAlloc<int32_t>(static_cast<size_t>(kMaxSlotsPerBlock) * kMaxSelect * 2);
-  sel_count = DevAlloc<int32_t>(kMaxSlotsPerBlock);
+  sel = DevAlloc<int32_t>(static_cast<size_t>(kMaxWindowSlots) * kMaxSelect * 2);
+  sel_count = DevAlloc<int32_t>(kMaxWindowSlots);
+  slot_stats = DevAlloc<long long>(static_cast<size_t>(kMaxWindowSlots) * kSlotStats);
+  touch = DevAlloc<unsigned long long>(n);
+  commit_out = DevAlloc<int32_t>(2);
+  done = DevAlloc<unsigned int>(1);
   mark_dest = DevAlloc<int32_t>(n);
   dest = DevAlloc<int32_t>(n);
   mark_keep = DevAlloc<int32_t>(n);
@@ -576,7 +588,10 @@ void Engine::Reset() {
   HIP_CHECK(hipMemsetAsync(tops, 0, 2 * sizeof(unsigned long long), stream));
   HIP_CHECK(hipMemsetAsync(ctr, 0, sizeof(Counters) * (kMaxIterations + 1), stream));
   HIP_CHECK(hipMemsetAsync(err, 0, sizeof(int), stream));
+  HIP_CHECK(hipMemsetAsync(touch, 0, sizeof(unsigned long long) * n, stream));
+  HIP_CHECK(hipMemsetAsync(done, 0, sizeof(unsigned int), stream));
   epoch = 0;
+  windows_run = 0;
   blocks_run = 0;
   divide_max_group = 0;
   divide_fallback_splits = 0;
@@ -710,9 +725,13 @@ int Engine::Divide(int iter) {
   return G;
 }
 
-void Engine::LaunchBlock(int first_group, int nslots, int ngroups, double theta, Counters* c, const int32_t* order_arr,
-                         const int32_t* gstart_arr, const int32_t* glen_arr, int32_t* dbg_pairs, long long* dbg_gain,
-                         int32_t* dbg_count, int dbg_cap, bool commit) {
+// Scores the groups [first_group, first_group + nslots) speculatively on the
+// current state (one workgroup per group), then -- with commit -- resolves how
+// many leading blocks of block_size groups can be committed together and
+// commits them.  Returns the number of committed blocks (0 without commit).
+int Engine::LaunchWindow(int first_group, int nslots, int ngroups, int block_size, double theta, Counters* c,
+                         const int32_t* order_arr, const int32_t* gstart_arr, const int32_t* glen_arr,
+                         int32_t* dbg_pairs, long long* dbg_gain, int32_t* dbg_count, int dbg_cap, bool commit) {
   ++epoch;
   MergeParams mp{};
   mp.q = QuotientView{pool[pool_idx], row_off, row_len, size, internal};
@@ -723,17 +742,11 @@ void Engine::LaunchBlock(int first_group, int nslots, int ngroups, double theta,
   mp.first_group = first_group;
   mp.top_k = opt.top_k;
   mp.theta = theta;
-  mp.epoch = epoch;
   mp.sel = sel;
   mp.sel_count = sel_count;
-  mp.mark_dest = mark_dest;
-  mp.dest = dest;
-  mp.mark_keep = mark_keep;
-  mp.parent = parent;
-  mp.mark_row = mark_row;
-  mp.aff = aff;
-  mp.naff = &naff[epoch & 1];
+  mp.slot_stats = slot_stats;
   mp.reset_top = &tops[1];
+  mp.reset_queue = &naff[epoch & 1];
   mp.scratch = scratch_a;
   mp.scratch_cap = scratch_a_cap;
   mp.scratch_top = &tops[0];
@@ -744,7 +757,6 @@ void Engine::LaunchBlock(int first_group, int nslots, int ngroups, double theta,
   mp.dbg_gain = dbg_gain;
   mp.dbg_count = dbg_count;
   mp.dbg_cap = dbg_cap;
-  mp.commit = commit ? 1 : 0;
   if (profile) {
     if (ev_used + 2 > ev_pool.size()) {
       for (int k = 0; k < 256; ++k) {
@@ -757,11 +769,49 @@ void Engine::LaunchBlock(int first_group, int nslots, int ngroups, double theta,
   }
   MergeGroupsKernel<<<nslots, 512, 0, stream>>>(mp);
   if (profile) HIP_CHECK(hipEventRecord(ev_pool[ev_used++], stream));
-  ++blocks_run;
-  if (commit) Commit(nslots, c);
+  ++windows_run;
+  if (!commit) {
+    // No commit zeroes the group scratch top for the next launch.
+    HIP_CHECK(hipMemsetAsync(&tops[0], 0, sizeof(unsigned long long), stream));
+    return 0;
+  }
+
+  ResolveParams rp{};
+  rp.q = mp.q;
+  rp.order = order_arr;
+  rp.gstart = gstart_arr;
+  rp.glen = glen_arr;
+  rp.ngroups = ngroups;
+  rp.first_group = first_group;
+  rp.nslots = nslots;
+  rp.block_size = block_size;
+  rp.epoch = epoch;
+  rp.sel = sel;
+  rp.sel_count = sel_count;
+  rp.slot_stats = slot_stats;
+  rp.touch = touch;
+  rp.mark_dest = mark_dest;
+  rp.dest = dest;
+  rp.mark_keep = mark_keep;
+  rp.parent = parent;
+  rp.mark_row = mark_row;
+  rp.aff = aff;
+  rp.naff = &naff[epoch & 1];
+  rp.commit_slots = &commit_out[0];
+  rp.commit_blocks = &commit_out[1];
+  rp.done = done;
+  rp.ctr = c;
+  rp.err = err;
+  ResolveKernel<<<kResolveGrid, 512, 0, stream>>>(rp);
+  Commit(c);
+  int32_t blocks = 0;
+  HIP_CHECK(hipMemcpyAsync(&blocks, &commit_out[1], sizeof(blocks), hipMemcpyDeviceToHost, stream));
+  Sync();
+  blocks_run += blocks;
+  return blocks;
 }
 
-void Engine::Commit(int nslots, Counters* c) {
+void Engine::Commit(Counters* c) {
   CommitParams cp{};
   cp.pool = pool[pool_idx];
   cp.row_off = row_off;
@@ -772,8 +822,8 @@ void Engine::Commit(int nslots, Counters* c) {
   cp.active = active;
   cp.sel = sel;
   cp.sel_count = sel_count;
-  cp.nslots = nslots;
-  cp.epoch = epoch;
+  cp.nslots = &commit_out[0];
+  cp.epoch = static_cast<int>(epoch);
   cp.mark_dest = mark_dest;
   cp.dest = dest;
   cp.mark_keep = mark_keep;
@@ -822,9 +872,19 @@ void Engine::Iterate(int iter) {
     trace_groups = G;
   }
   const int bs = opt.group_batch_size;
-  for (int first = 0; first < G; first += bs) {
-    const int nslots = std::min(bs, G - first);
-    LaunchBlock(first, nslots, G, theta, c, order, gstart, glen, nullptr, nullptr, nullptr, 0, true);
+  const int nblocks = (G + bs - 1) / bs;
+  const int wb = std::max(1, std::min(window_blocks, kMaxWindowSlots / bs));
+  int cursor = 0;
+  while (cursor < nblocks) {
+    const int first = cursor * bs;
+    const int nslots = std::min(wb * bs, G - first);
+    const int committed = LaunchWindow(first, nslots, G, bs, theta, c, order, gstart, glen, nullptr, nullptr, nullptr, 0,
+                                       true);
+    if (committed <= 0) {
+      CheckDeviceError("merge");
+      throw std::runtime_error("window resolution committed no block");
+    }
+    cursor += committed;
   }
   Sync();
   tracing_now = false;
@@ -971,6 +1031,7 @@ int beam_hip_stats(beam_hip_engine* engine, int64_t* out) {
     out[BEAM_HIP_STAT_VALIDATION_EXACT_ROW_ENTRY_WORK] = static_cast<int64_t>(t.validation_row_entry_work);
     out[BEAM_HIP_STAT_ISOLATED_GAIN_SUM] = t.total_local_gain;
     out[BEAM_HIP_STAT_BLOCKS] = e.blocks_run;
+    out[BEAM_HIP_STAT_WINDOWS] = e.windows_run;
     return 0;
   });
 }
@@ -1170,6 +1231,7 @@ int beam_hip_merge_groups(beam_hip_engine* engine, const int32_t* reps, int32_t 
     auto& e = engine->e;
     if (ngroups <= 0) return 0;
     if (ngroups > beam::kMaxSlotsPerBlock) throw std::invalid_argument("at most 1024 groups per call");
+    HIP_CHECK(hipMemsetAsync(e.ctr, 0, sizeof(beam::Counters), e.stream));
     int32_t* dreps = beam::DevAlloc<int32_t>(static_cast<size_t>(nreps));
     int32_t* dst = beam::DevAlloc<int32_t>(static_cast<size_t>(ngroups));
     int32_t* dln = beam::DevAlloc<int32_t>(static_cast<size_t>(ngroups));
@@ -1181,7 +1243,8 @@ int beam_hip_merge_groups(beam_hip_engine* engine, const int32_t* reps, int32_t 
     HIP_CHECK(hipMemcpyAsync(dln, lengths, sizeof(int32_t) * ngroups, hipMemcpyHostToDevice, e.stream));
     HIP_CHECK(hipMemsetAsync(dcount, 0, sizeof(int32_t) * ngroups, e.stream));
     beam::Counters* c = e.ctr;  // slot 0 collects counters of per-stage calls
-    e.LaunchBlock(0, ngroups, ngroups, theta, c, dreps, dst, dln, dpairs, dgain, dcount, cand_cap, commit != 0);
+    e.LaunchWindow(0, ngroups, ngroups, ngroups, theta, c, dreps, dst, dln, dpairs, dgain, dcount, cand_cap,
+                   commit != 0);
     HIP_CHECK(hipMemcpyAsync(cand_count, dcount, sizeof(int32_t) * ngroups, hipMemcpyDeviceToHost, e.stream));
     HIP_CHECK(hipMemcpyAsync(cand_pairs, dpairs, sizeof(int32_t) * ngroups * cand_cap * 2, hipMemcpyDeviceToHost, e.stream));
     HIP_CHECK(hipMemcpyAsync(cand_gain, dgain, sizeof(long long) * ngroups * cand_cap * 2, hipMemcpyDeviceToHost, e.stream));
@@ -1217,7 +1280,7 @@ int beam_hip_commit(beam_hip_engine* engine, const int32_t* keeps, const int32_t
   return Guard([&] {
     auto& e = engine->e;
     if (count <= 0) return 0;
-    const int max_pairs = beam::kMaxSlotsPerBlock * beam::kMaxSelect;
+    const int max_pairs = beam::kMaxWindowSlots * beam::kMaxSelect;
     if (count > max_pairs) throw std::invalid_argument("too many pairs in one commit");
     const int nslots = (count + beam::kMaxSelect - 1) / beam::kMaxSelect;
     std::vector<int32_t> hsel(static_cast<size_t>(nslots) * beam::kMaxSelect * 2, 0);
@@ -1232,10 +1295,12 @@ int beam_hip_commit(beam_hip_engine* engine, const int32_t* keeps, const int32_t
     HIP_CHECK(hipMemcpyAsync(e.sel_count, hcount.data(), hcount.size() * sizeof(int32_t), hipMemcpyHostToDevice, e.stream));
     ++e.epoch;
     beam::MarkCommit<<<nslots, 256, 0, e.stream>>>(
-        e.sel, e.sel_count, nslots, e.epoch, e.mark_dest, e.dest, e.mark_keep, e.parent,
+        e.sel, e.sel_count, nslots, static_cast<int>(e.epoch), e.mark_dest, e.dest, e.mark_keep, e.parent,
         beam::QuotientView{e.pool[e.pool_idx], e.row_off, e.row_len, e.size, e.internal}, e.mark_row, e.aff,
         &e.naff[e.epoch & 1]);
-    e.Commit(nslots, e.ctr);
+    const int32_t counts[2] = {nslots, 1};
+    HIP_CHECK(hipMemcpyAsync(e.commit_out, counts, sizeof(counts), hipMemcpyHostToDevice, e.stream));
+    e.Commit(e.ctr);
     e.Sync();
     e.CheckDeviceError("commit");
     return 0;

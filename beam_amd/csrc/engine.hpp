@@ -24,6 +24,22 @@ namespace beam {
 
 constexpr int kMaxSelect = 256;          // selected pairs per group (<= 512 / 2)
 constexpr int kMaxSlotsPerBlock = 1024;  // groups per block (--group-batch-size)
+constexpr int kMaxWindowSlots = 4096;    // groups scored speculatively per launch
+
+// Per-slot (group) statistics written by the group kernel.
+enum : int {
+  kStatGroup = 0,   // group size g (0: singleton or no group)
+  kStatCand,        // candidate pairs after top-k
+  kStatProxy,       // bucket pairs examined
+  kStatPositive,    // candidates with gain > 0
+  kStatPass,        // candidates passing the threshold
+  kStatOverlap,     // survivors rejected by overlap
+  kStatChosen,      // selected pairs
+  kStatGain,        // sum of selected gains
+  kStatWork,        // row entries of selected pairs (validation work)
+  kStatEntries,     // cross entries of the member rows
+  kSlotStats = 12
+};
 
 struct QuotientView {
   const Entry* pool;
@@ -42,21 +58,14 @@ struct MergeParams {
   int first_group;        // first group of this block
   int top_k;
   double theta;           // 1.0 / (iter + 1), computed on the host
-  int epoch;              // unique per block; tags dest / keep marks
-  int commit;             // record selections as merges of this block
   int32_t* sel;           // [slot][kMaxSelect][2] selected (keep, remove)
   int32_t* sel_count;     // [slot]
-  int32_t* mark_dest;     // epoch tag: rep is a remove of this block
-  int32_t* dest;          // keep for a tagged remove
-  int32_t* mark_keep;     // epoch tag: rep is a keep of this block
-  int32_t* parent;
-  int32_t* mark_row;      // epoch tag: neighbour row already queued
-  int32_t* aff;           // queued neighbour rows of the block's removes
-  int32_t* naff;
+  long long* slot_stats;  // [slot][kSlotStats]
   unsigned char* scratch;
   unsigned long long scratch_cap;
   unsigned long long* scratch_top;
   unsigned long long* reset_top;  // the commit's scratch top, zeroed here
+  int32_t* reset_queue;           // this window's neighbour queue length, zeroed here
   Counters* ctr;
   int* err;
   // Optional phase trace: 16 values per group at trace[16 * group]
@@ -69,6 +78,36 @@ struct MergeParams {
   int dbg_cap;
 };
 
+// Resolution of a speculative window: which leading blocks can be committed
+// together (ResolveKernel), then their merges marked and queued.
+struct ResolveParams {
+  QuotientView q;
+  const int32_t* order;
+  const int32_t* gstart;
+  const int32_t* glen;
+  int ngroups;
+  int first_group;        // first group of the window
+  int nslots;             // groups in the window
+  int block_size;         // groups per block
+  long long epoch;        // unique per window; tags touches and merge marks
+  const int32_t* sel;
+  const int32_t* sel_count;
+  const long long* slot_stats;
+  unsigned long long* touch;  // (epoch << 16) | (0xffff - first touching block)
+  int32_t* mark_dest;
+  int32_t* dest;
+  int32_t* mark_keep;
+  int32_t* parent;
+  int32_t* mark_row;
+  int32_t* aff;
+  int32_t* naff;
+  int32_t* commit_slots;  // out: slots committed (whole blocks)
+  int32_t* commit_blocks; // out: blocks committed
+  unsigned int* done;     // workgroups finished (reset by the last one)
+  Counters* ctr;
+  int* err;
+};
+
 struct CommitParams {
   Entry* pool;
   uint32_t* row_off;
@@ -79,7 +118,7 @@ struct CommitParams {
   unsigned char* active;
   const int32_t* sel;
   const int32_t* sel_count;
-  int nslots;
+  const int32_t* nslots;       // slots to commit (device)
   int epoch;
   const int32_t* mark_dest;
   const int32_t* dest;
@@ -98,6 +137,7 @@ struct CommitParams {
 };
 
 __global__ void MergeGroupsKernel(MergeParams p);
+__global__ void ResolveKernel(ResolveParams p);
 __global__ void CommitKernel(CommitParams p);
 
 }  // namespace beam

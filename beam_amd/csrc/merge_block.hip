@@ -329,7 +329,9 @@ __global__ __launch_bounds__(kThreads) void MergeGroupsKernel(MergeParams p) {
   const int tid = static_cast<int>(threadIdx.x);
   if (tid == 0) {
     p.sel_count[slot] = 0;
+    p.slot_stats[static_cast<long long>(slot) * kSlotStats + kStatGroup] = 0;
     if (slot == 0 && p.reset_top != nullptr) *p.reset_top = 0;
+    if (slot == 0 && p.reset_queue != nullptr) *p.reset_queue = 0;
   }
   if (gidx >= p.ngroups) return;
   const int g = p.glen[gidx];
@@ -360,9 +362,8 @@ __global__ __launch_bounds__(kThreads) void MergeGroupsKernel(MergeParams p) {
   __shared__ int16_t s_ent_i[kSmallE];
   __shared__ __align__(16) unsigned char s_region[kRegionBytes];
   __shared__ unsigned char s_used[kMaxGroup];
-  __shared__ int16_t s_sel[kMaxSelect * 2];
   __shared__ unsigned long long s_base;
-  __shared__ int s_flag, s_chosen;
+  __shared__ int s_flag;
   __shared__ uint32_t s_counts[2];
 
   // Phase 1: members, member lookup table, staged entry offsets.
@@ -795,38 +796,26 @@ __global__ __launch_bounds__(kThreads) void MergeGroupsKernel(MergeParams p) {
       const int keep = m_rep[a], remove = m_rep[b];
       p.sel[2 * (slot * kMaxSelect + chosen)] = keep;
       p.sel[2 * (slot * kMaxSelect + chosen) + 1] = remove;
-      s_sel[2 * chosen] = static_cast<int16_t>(a);
-      s_sel[2 * chosen + 1] = static_cast<int16_t>(b);
-      if (p.commit) {
-        p.mark_dest[remove] = p.epoch;
-        p.dest[remove] = keep;
-        p.mark_keep[keep] = p.epoch;
-        p.parent[remove] = keep;
-      }
       work += static_cast<unsigned long long>(m_len[a] + m_len[b]);
       total += A.cand_gain[px];
       ++chosen;
     }
     p.sel_count[slot] = chosen;
-    s_chosen = chosen;
-    Counters* c = p.ctr;
+    // Per-slot statistics; the window resolve adds them to the iteration
+    // counters only for blocks that are committed.
     unsigned long long entries = 0;
     for (int i = 0; i < g; ++i) entries += static_cast<unsigned long long>(m_len[i]);
-    atomicAdd(&c->group_count, 1ULL);
-    atomicMax(&c->group_max_size, static_cast<unsigned long long>(g));
-    atomicAdd(&c->raw_pair_count, static_cast<unsigned long long>(g) * (g - 1) / 2);
-    atomicAdd(&c->candidate_pairs_after_prune, static_cast<unsigned long long>(C));
-    atomicAdd(&c->proxy_pairs_examined, static_cast<unsigned long long>(PT));
-    atomicAdd(&c->exact_gain_calls, static_cast<unsigned long long>(C));
-    atomicAdd(&c->positive_gain_count, static_cast<unsigned long long>(s_counts[0]));
-    atomicAdd(&c->above_threshold, static_cast<unsigned long long>(P));
-    atomicAdd(&c->rejected_by_threshold, static_cast<unsigned long long>(C - P));
-    atomicAdd(&c->rejected_by_overlap, overlap);
-    atomicAdd(&c->selected, static_cast<unsigned long long>(chosen));
-    atomicAdd(reinterpret_cast<unsigned long long*>(&c->total_local_gain), static_cast<unsigned long long>(total));
-    atomicAdd(&c->validation_row_entry_work, work);
-    atomicAdd(&c->member_rows, static_cast<unsigned long long>(g));
-    atomicAdd(&c->member_entries, entries);
+    long long* st = p.slot_stats + static_cast<long long>(slot) * kSlotStats;
+    st[kStatGroup] = g;
+    st[kStatCand] = C;
+    st[kStatProxy] = PT;
+    st[kStatPositive] = s_counts[0];
+    st[kStatPass] = P;
+    st[kStatOverlap] = static_cast<long long>(overlap);
+    st[kStatChosen] = chosen;
+    st[kStatGain] = total;
+    st[kStatWork] = static_cast<long long>(work);
+    st[kStatEntries] = static_cast<long long>(entries);
   }
   __syncthreads();
 
@@ -836,21 +825,6 @@ __global__ __launch_bounds__(kThreads) void MergeGroupsKernel(MergeParams p) {
     trace[2] = PT;
     trace[3] = C;
     trace[4] = P;
-  }
-  // Phase 12: queue the neighbours of every remove for re-pointing; rows
-  // that turn out to be endpoints of this block are skipped by the commit.
-  if (p.commit) {
-    const int chosen = s_chosen;
-    for (int k = 0; k < chosen; ++k) {
-      const int b = s_sel[2 * k + 1];
-      const int keep = m_rep[s_sel[2 * k]];
-      const uint32_t off = m_eoff[b];
-      for (int e = tid; e < m_len[b]; e += kThreads) {
-        const int x = A.ent_t[off + e];
-        if (x == keep) continue;
-        if (atomicExch(&p.mark_row[x], p.epoch) != p.epoch) p.aff[atomicAdd(p.naff, 1)] = x;
-      }
-    }
   }
   __syncthreads();
   stamp(13);

@@ -72,8 +72,9 @@ int beam_hip_iterate(beam_hip_engine* engine, int32_t iter);
 int beam_hip_labels(beam_hip_engine* engine, int32_t* labels);
 
 /* Summary counters, BEAM_HIP_STAT_COUNT entries in the order of the
- * BEAM_HIP_STAT_* indices (RuntimeStats, include/sweg.hpp:221-386). */
-#define BEAM_HIP_STAT_COUNT 20
+ * BEAM_HIP_STAT_* indices (RuntimeStats, include/sweg.hpp:221-386); the last
+ * two count the blocks merged and the speculative windows launched. */
+#define BEAM_HIP_STAT_COUNT 21
 enum {
   BEAM_HIP_STAT_DIVIDE_MAX_GROUP_SIZE = 0,
   BEAM_HIP_STAT_DIVIDE_FALLBACK_SPLITS,
@@ -94,7 +95,8 @@ enum {
   BEAM_HIP_STAT_VALIDATION_EXACT_CALLS,
   BEAM_HIP_STAT_VALIDATION_EXACT_ROW_ENTRY_WORK,
   BEAM_HIP_STAT_ISOLATED_GAIN_SUM,
-  BEAM_HIP_STAT_BLOCKS
+  BEAM_HIP_STAT_BLOCKS,
+  BEAM_HIP_STAT_WINDOWS
 };
 int beam_hip_stats(beam_hip_engine* engine, int64_t* out);
 
