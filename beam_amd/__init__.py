@@ -93,6 +93,7 @@ def library() -> ctypes.CDLL:
         "beam_hip_run": (ctypes.c_int, [P, i32]),
         "beam_hip_iterate": (ctypes.c_int, [P, i32]),
         "beam_hip_labels": (ctypes.c_int, [P, P]),
+        "beam_hip_supernodes": (ctypes.c_int, [P, P, P]),
         "beam_hip_stats": (ctypes.c_int, [P, P]),
         "beam_hip_round_stats": (ctypes.c_int, [P, P, i32]),
         "beam_hip_timing": (ctypes.c_int, [P, P]),
@@ -131,7 +132,7 @@ def exported_symbols() -> list[str]:
     """Names of the C entry points this wrapper binds."""
     return [
         "beam_hip_build_mode", "beam_hip_compiled", "beam_hip_last_error", "beam_hip_create",
-        "beam_hip_destroy", "beam_hip_run", "beam_hip_iterate", "beam_hip_labels", "beam_hip_stats",
+        "beam_hip_destroy", "beam_hip_run", "beam_hip_iterate", "beam_hip_labels", "beam_hip_supernodes", "beam_hip_stats",
         "beam_hip_round_stats", "beam_hip_timing", "beam_hip_encoding_counts", "beam_hip_upload_state",
         "beam_hip_nnz", "beam_hip_download_state", "beam_hip_score_pairs", "beam_hip_merge_groups",
         "beam_hip_divide", "beam_hip_commit", "beam_hip_reset", "beam_hip_profile",
@@ -234,6 +235,15 @@ class Engine:
         out = np.empty(self.n, dtype=np.int32)
         _check(self.lib.beam_hip_labels(self.h, _ptr(out)))
         return out
+
+    def supernodes(self) -> tuple[np.ndarray, np.ndarray]:
+        """(offsets, members): supernodes by ascending representative, members
+        in the reference's Recover_S order."""
+        offsets = np.empty(self.n + 1, dtype=np.int32)
+        members = np.empty(self.n, dtype=np.int32)
+        count = self.lib.beam_hip_supernodes(self.h, _ptr(offsets), _ptr(members))
+        _check(count)
+        return offsets[: count + 1].copy(), members
 
     def stats(self) -> dict:
         out = np.zeros(len(STAT_NAMES), dtype=np.int64)

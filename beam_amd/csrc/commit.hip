@@ -350,6 +350,7 @@ __global__ __launch_bounds__(kThreads) void ResolveKernel(ResolveParams p) {
     p.dest[remove] = keep;
     p.mark_keep[keep] = static_cast<int32_t>(p.epoch);
     p.parent[remove] = keep;
+    p.merge_seq[remove] = static_cast<int32_t>(p.epoch);
   }
   // Neighbours of every committed remove are queued for re-pointing.
   for (uint32_t item = tid; item < npairs; item += kThreads) {

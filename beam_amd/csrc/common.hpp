@@ -4,6 +4,19 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <stdexcept>
+#include <string>
+
+// Host-side status check: a failed HIP call throws, and the C-ABI turns the
+// exception into a negative status and beam_hip_last_error().
+#define HIP_CHECK(expr)                                                                     \
+  do {                                                                                      \
+    hipError_t _e = (expr);                                                                 \
+    if (_e != hipSuccess) {                                                                 \
+      throw std::runtime_error(std::string(#expr) + " failed: " + hipGetErrorString(_e));   \
+    }                                                                                       \
+  } while (0)
+
 namespace beam {
 
 constexpr int kWave = 64;

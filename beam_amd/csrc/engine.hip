@@ -19,14 +19,6 @@ namespace {
 
 thread_local std::string g_error;
 
-#define HIP_CHECK(expr)                                                                     \
-  do {                                                                                      \
-    hipError_t _e = (expr);                                                                 \
-    if (_e != hipSuccess) {                                                                 \
-      throw std::runtime_error(std::string(#expr) + " failed: " + hipGetErrorString(_e));   \
-    }                                                                                       \
-  } while (0)
-
 constexpr int kMaxIterations = 4096;
 constexpr int kCommitGrid = 512;
 constexpr int kResolveGrid = 256;
@@ -335,8 +327,8 @@ __global__ __launch_bounds__(256) void ScorePairsKernel(QuotientView q, const in
 // queues the neighbours of every remove (the group kernel does both for the
 // pairs it selects).
 __global__ void MarkCommit(const int32_t* sel, const int32_t* sel_count, int nslots, int epoch, int32_t* mark_dest,
-                           int32_t* dest, int32_t* mark_keep, int32_t* parent, QuotientView q, int32_t* mark_row,
-                           int32_t* aff, int32_t* naff) {
+                           int32_t* dest, int32_t* mark_keep, int32_t* parent, int32_t* merge_seq, QuotientView q,
+                           int32_t* mark_row, int32_t* aff, int32_t* naff) {
   const int s = blockIdx.x;
   if (s >= nslots) return;
   for (int k = threadIdx.x; k < sel_count[s]; k += blockDim.x) {
@@ -345,6 +337,7 @@ __global__ void MarkCommit(const int32_t* sel, const int32_t* sel_count, int nsl
     dest[remove] = keep;
     mark_keep[keep] = epoch;
     parent[remove] = keep;
+    merge_seq[remove] = epoch;
     const Entry* row = q.pool + q.row_off[remove];
     for (int e = 0; e < q.row_len[remove]; ++e) {
       const int x = row[e].nbr;
@@ -368,6 +361,7 @@ struct Engine {
   int32_t* col = nullptr;
   int32_t* label = nullptr;
   int32_t* parent = nullptr;
+  int32_t* merge_seq = nullptr;  // window epoch at which r was absorbed (0: never)
   unsigned char* active = nullptr;
 
   Entry* pool[2] = {nullptr, nullptr};
@@ -469,8 +463,8 @@ struct Engine {
 };
 
 Engine::~Engine() {
-  if (stream) hipStreamSynchronize(stream);
-  void* ptrs[] = {row_ptr,  col,      label,    parent,    active,   pool[0],    pool[1],   pool_top, row_off,
+  if (stream) (void)hipStreamSynchronize(stream);
+  void* ptrs[] = {row_ptr,  col,      label,    parent,    merge_seq, active,   pool[0],    pool[1],   pool_top, row_off,
                   row_len,  row_cap,  size,     internal,  order,    open_pos[0], open_pos[1], seg[0], seg[1],
                   open_rep, mh,       recs[0],  recs[1],   flag,     excl,       rid,       run_start, run_open,
                   run_seg,  elem_open, new_idx, gflag,     gstart,   glen,       d_total,   d_maxlen, sel,
@@ -506,6 +500,7 @@ void Engine::Create(const int64_t* h_row_ptr, const int32_t* h_col, int n_, long
   HIP_CHECK(hipMemcpyAsync(col, h_col, static_cast<size_t>(arcs) * sizeof(int32_t), hipMemcpyHostToDevice, stream));
   label = DevAlloc<int32_t>(n);
   parent = DevAlloc<int32_t>(n);
+  merge_seq = DevAlloc<int32_t>(n);
   active = DevAlloc<unsigned char>(n);
   // Row storage: the live rows never exceed the initial arcs; each
   // iteration's unions need at most that much fresh space, so two and a half
@@ -582,6 +577,7 @@ void Engine::Reset() {
   const unsigned long long top = static_cast<unsigned long long>(arcs);
   HIP_CHECK(hipMemcpyAsync(pool_top, &top, sizeof(top), hipMemcpyHostToDevice, stream));
   HIP_CHECK(hipMemsetAsync(mark_dest, 0, sizeof(int32_t) * n, stream));
+  HIP_CHECK(hipMemsetAsync(merge_seq, 0, sizeof(int32_t) * n, stream));
   HIP_CHECK(hipMemsetAsync(mark_keep, 0, sizeof(int32_t) * n, stream));
   HIP_CHECK(hipMemsetAsync(mark_row, 0, sizeof(int32_t) * n, stream));
   HIP_CHECK(hipMemsetAsync(naff, 0, 2 * sizeof(int32_t), stream));
@@ -794,6 +790,7 @@ int Engine::LaunchWindow(int first_group, int nslots, int ngroups, int block_siz
   rp.dest = dest;
   rp.mark_keep = mark_keep;
   rp.parent = parent;
+  rp.merge_seq = merge_seq;
   rp.mark_row = mark_row;
   rp.aff = aff;
   rp.naff = &naff[epoch & 1];
@@ -990,6 +987,59 @@ int beam_hip_labels(beam_hip_engine* engine, int32_t* labels) {
   });
 }
 
+int beam_hip_supernodes(beam_hip_engine* engine, int32_t* offsets, int32_t* members) {
+  return Guard([&] {
+    auto& e = engine->e;
+    const int n = e.n;
+    std::vector<int32_t> parent(static_cast<size_t>(n)), seq(static_cast<size_t>(n));
+    HIP_CHECK(hipMemcpyAsync(parent.data(), e.parent, sizeof(int32_t) * n, hipMemcpyDeviceToHost, e.stream));
+    HIP_CHECK(hipMemcpyAsync(seq.data(), e.merge_seq, sizeof(int32_t) * n, hipMemcpyDeviceToHost, e.stream));
+    e.Sync();
+    // Children of every supernode in the order they were absorbed: one merge
+    // per supernode per iteration, so the window epoch orders them; ties
+    // (uploaded states) fall back to ascending node id.
+    std::vector<int32_t> first(static_cast<size_t>(n) + 1, 0), child(static_cast<size_t>(n));
+    for (int v = 0; v < n; ++v) {
+      const int p = parent[static_cast<size_t>(v)];
+      if (p < 0 || p >= n) throw std::runtime_error("corrupt merge forest");
+      if (p != v) ++first[static_cast<size_t>(p) + 1];
+    }
+    for (int v = 0; v < n; ++v) first[static_cast<size_t>(v) + 1] += first[static_cast<size_t>(v)];
+    {
+      std::vector<int32_t> fill(first.begin(), first.end() - 1);
+      for (int v = 0; v < n; ++v) {
+        const int p = parent[static_cast<size_t>(v)];
+        if (p != v) child[static_cast<size_t>(fill[static_cast<size_t>(p)]++)] = v;
+      }
+    }
+    for (int v = 0; v < n; ++v) {
+      std::stable_sort(child.begin() + first[static_cast<size_t>(v)], child.begin() + first[static_cast<size_t>(v) + 1],
+                       [&](int a, int b) { return seq[static_cast<size_t>(a)] < seq[static_cast<size_t>(b)]; });
+    }
+    // Recover_S order: the absorbing supernode's list, then each absorbed
+    // list in merge order -- a preorder walk of the merge tree.
+    int count = 0;
+    int32_t pos = 0;
+    std::vector<int32_t> stack;
+    for (int r = 0; r < n; ++r) {
+      if (parent[static_cast<size_t>(r)] != r) continue;
+      offsets[count++] = pos;
+      stack.push_back(r);
+      while (!stack.empty()) {
+        const int x = stack.back();
+        stack.pop_back();
+        if (pos >= n) throw std::runtime_error("corrupt merge forest");
+        members[pos++] = x;
+        for (int k = first[static_cast<size_t>(x) + 1] - 1; k >= first[static_cast<size_t>(x)]; --k)
+          stack.push_back(child[static_cast<size_t>(k)]);
+      }
+    }
+    if (pos != n) throw std::runtime_error("corrupt merge forest");
+    offsets[count] = pos;
+    return count;
+  });
+}
+
 int beam_hip_stats(beam_hip_engine* engine, int64_t* out) {
   return Guard([&] {
     auto& e = engine->e;
@@ -1154,6 +1204,7 @@ int beam_hip_upload_state(beam_hip_engine* engine, const int64_t* offsets, const
     HIP_CHECK(hipMemcpyAsync(e.active, active, n, hipMemcpyHostToDevice, e.stream));
     HIP_CHECK(hipMemcpyAsync(e.label, labels, sizeof(int32_t) * n, hipMemcpyHostToDevice, e.stream));
     HIP_CHECK(hipMemcpyAsync(e.parent, parent.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, e.stream));
+    HIP_CHECK(hipMemsetAsync(e.merge_seq, 0, sizeof(int32_t) * n, e.stream));
     const unsigned long long top = static_cast<unsigned long long>(total);
     HIP_CHECK(hipMemcpyAsync(e.pool_top, &top, sizeof(top), hipMemcpyHostToDevice, e.stream));
     e.Sync();
@@ -1214,10 +1265,10 @@ int beam_hip_score_pairs(beam_hip_engine* engine, const int32_t* a, const int32_
     HIP_CHECK(hipMemcpyAsync(gain, dg, sizeof(long long) * count, hipMemcpyDeviceToHost, e.stream));
     HIP_CHECK(hipMemcpyAsync(before, dbf, sizeof(long long) * count, hipMemcpyDeviceToHost, e.stream));
     e.Sync();
-    hipFree(da);
-    hipFree(db);
-    hipFree(dg);
-    hipFree(dbf);
+    (void)hipFree(da);
+    (void)hipFree(db);
+    (void)hipFree(dg);
+    (void)hipFree(dbf);
     e.CheckDeviceError("score_pairs");
     return 0;
   });
@@ -1251,12 +1302,12 @@ int beam_hip_merge_groups(beam_hip_engine* engine, const int32_t* reps, int32_t 
     HIP_CHECK(hipMemcpyAsync(sel_count, e.sel_count, sizeof(int32_t) * ngroups, hipMemcpyDeviceToHost, e.stream));
     HIP_CHECK(hipMemcpyAsync(sel_pairs, e.sel, sizeof(int32_t) * ngroups * beam::kMaxSelect * 2, hipMemcpyDeviceToHost, e.stream));
     e.Sync();
-    hipFree(dreps);
-    hipFree(dst);
-    hipFree(dln);
-    hipFree(dpairs);
-    hipFree(dgain);
-    hipFree(dcount);
+    (void)hipFree(dreps);
+    (void)hipFree(dst);
+    (void)hipFree(dln);
+    (void)hipFree(dpairs);
+    (void)hipFree(dgain);
+    (void)hipFree(dcount);
     e.CheckDeviceError("merge_groups");
     return 0;
   });
@@ -1296,7 +1347,7 @@ int beam_hip_commit(beam_hip_engine* engine, const int32_t* keeps, const int32_t
     ++e.epoch;
     beam::MarkCommit<<<nslots, 256, 0, e.stream>>>(
         e.sel, e.sel_count, nslots, static_cast<int>(e.epoch), e.mark_dest, e.dest, e.mark_keep, e.parent,
-        beam::QuotientView{e.pool[e.pool_idx], e.row_off, e.row_len, e.size, e.internal}, e.mark_row, e.aff,
+        e.merge_seq, beam::QuotientView{e.pool[e.pool_idx], e.row_off, e.row_len, e.size, e.internal}, e.mark_row, e.aff,
         &e.naff[e.epoch & 1]);
     const int32_t counts[2] = {nslots, 1};
     HIP_CHECK(hipMemcpyAsync(e.commit_out, counts, sizeof(counts), hipMemcpyHostToDevice, e.stream));

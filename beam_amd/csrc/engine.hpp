@@ -98,6 +98,7 @@ struct ResolveParams {
   int32_t* dest;
   int32_t* mark_keep;
   int32_t* parent;
+  int32_t* merge_seq;     // window epoch of each committed remove
   int32_t* mark_row;
   int32_t* aff;
   int32_t* naff;

@@ -110,34 +110,6 @@ __device__ __forceinline__ void TriDecode(uint32_t l, uint32_t s, uint32_t* x, u
   *y = static_cast<uint32_t>(static_cast<int64_t>(l) - row_start(xi) + xi + 1);
 }
 
-// Candidate ranking key (sweg.cpp:1926-1938): ratio desc, proxy desc,
-// rep_lo asc, rep_hi asc.
-struct RankKey {
-  double ratio;
-  long long proxy;
-  int lo, hi;
-  int valid;
-};
-
-__device__ __forceinline__ bool Better(const RankKey& l, const RankKey& r) {
-  if (l.valid != r.valid) return l.valid > r.valid;
-  if (!l.valid) return false;
-  if (l.ratio != r.ratio) return l.ratio > r.ratio;
-  if (l.proxy != r.proxy) return l.proxy > r.proxy;
-  if (l.lo != r.lo) return l.lo < r.lo;
-  return l.hi < r.hi;
-}
-
-__device__ __forceinline__ RankKey ShflKey(const RankKey& k, int mask) {
-  RankKey o;
-  o.ratio = __shfl_xor(k.ratio, mask, kWave);
-  o.proxy = __shfl_xor(k.proxy, mask, kWave);
-  o.lo = __shfl_xor(k.lo, mask, kWave);
-  o.hi = __shfl_xor(k.hi, mask, kWave);
-  o.valid = __shfl_xor(k.valid, mask, kWave);
-  return o;
-}
-
 // Exact MAGS-compatible gain of members (a, b) from the staged rows, one wave
 // (quotient_graph.cpp:340-462).  The merge-join sums separate into per-row
 // streams plus a correction over common neighbours:

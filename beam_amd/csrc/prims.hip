@@ -152,7 +152,7 @@ __global__ void KeyBits(const SortRec* in, int n, unsigned long long* acc) {
 void DeviceScan(const uint32_t* in, uint32_t* out, int n, uint32_t* total, ScanTemp& tmp, hipStream_t stream) {
   const int tiles = (n + kScanTile - 1) / kScanTile;
   if (tiles == 0) {
-    hipMemsetAsync(total, 0, sizeof(uint32_t), stream);
+    HIP_CHECK(hipMemsetAsync(total, 0, sizeof(uint32_t), stream));
     return;
   }
   tmp.Ensure(static_cast<size_t>(tiles));
@@ -163,13 +163,15 @@ void DeviceScan(const uint32_t* in, uint32_t* out, int n, uint32_t* total, ScanT
 
 void ScanTemp::Ensure(size_t count) {
   if (count <= cap) return;
-  if (sums) hipFree(sums);
+  if (sums) (void)hipFree(sums);
+  sums = nullptr;
+  cap = 0;
+  HIP_CHECK(hipMalloc(&sums, count * 2 * sizeof(uint32_t)));
   cap = count * 2;
-  if (hipMalloc(&sums, cap * sizeof(uint32_t)) != hipSuccess) sums = nullptr;
 }
 
 ScanTemp::~ScanTemp() {
-  if (sums) hipFree(sums);
+  if (sums) (void)hipFree(sums);
 }
 
 // Sorts recs[0..n) stably by (seg, key); returns the buffer holding the result
@@ -180,11 +182,11 @@ SortRec* SegmentedKeySort(SortRec* recs, SortRec* tmp, int n, SortTemp& st, hipS
   const int tiles = (n + kSortTile - 1) / kSortTile;
   st.Ensure(static_cast<size_t>(tiles) * 256);
   unsigned long long init[3] = {0ULL, ~0ULL, 0ULL};
-  hipMemcpyAsync(st.bits, init, sizeof(init), hipMemcpyHostToDevice, stream);
+  HIP_CHECK(hipMemcpyAsync(st.bits, init, sizeof(init), hipMemcpyHostToDevice, stream));
   KeyBits<<<min(1024, (n + 255) / 256), 256, 0, stream>>>(recs, n, st.bits);
   unsigned long long bits[3];
-  hipMemcpyAsync(bits, st.bits, sizeof(bits), hipMemcpyDeviceToHost, stream);
-  hipStreamSynchronize(stream);
+  HIP_CHECK(hipMemcpyAsync(bits, st.bits, sizeof(bits), hipMemcpyDeviceToHost, stream));
+  HIP_CHECK(hipStreamSynchronize(stream));
   const unsigned long long varying = bits[0] & ~bits[1];
   SortRec* src = recs;
   SortRec* dst = tmp;
@@ -208,19 +210,21 @@ SortRec* SegmentedKeySort(SortRec* recs, SortRec* tmp, int n, SortTemp& st, hipS
 
 void SortTemp::Ensure(size_t count) {
   if (!bits) {
-    hipMalloc(&bits, 4 * sizeof(unsigned long long));
-    hipMalloc(&total, sizeof(uint32_t));
+    HIP_CHECK(hipMalloc(&bits, 4 * sizeof(unsigned long long)));
+    HIP_CHECK(hipMalloc(&total, sizeof(uint32_t)));
   }
   if (count <= cap) return;
-  if (hist) hipFree(hist);
+  if (hist) (void)hipFree(hist);
+  hist = nullptr;
+  cap = 0;
+  HIP_CHECK(hipMalloc(&hist, count * 2 * sizeof(uint32_t)));
   cap = count * 2;
-  if (hipMalloc(&hist, cap * sizeof(uint32_t)) != hipSuccess) hist = nullptr;
 }
 
 SortTemp::~SortTemp() {
-  if (hist) hipFree(hist);
-  if (bits) hipFree(bits);
-  if (total) hipFree(total);
+  if (hist) (void)hipFree(hist);
+  if (bits) (void)hipFree(bits);
+  if (total) (void)hipFree(total);
 }
 
 }  // namespace beam

@@ -1,1 +1,301 @@
-int main() { return 0; }
+// `beam` command line for the HIP backend: the reference's flags and metric
+// output (src/main.cpp:14-56, 327-423, 577-956) for the merge path this
+// backend implements, with the Divide + Merge loop on the GPU.
+//
+//   beam --input <edge-list> --merge-mode batch-ea-blocked --scoring-backend hip
+//        --cost-objective mags-compatible --state-backend persistent
+//        --candidate-index legacy --certification off --commit-policy g0
+//        --ea-use-threshold --threshold-policy reciprocal [--iterations 20]
+//        [--top-k 16] [--group-batch-size 64] [--seed S] [--write-output --out DIR]
+#include <algorithm>
+#include <chrono>
+#include <cstdint>
+#include <cstdio>
+#include <filesystem>
+#include <fstream>
+#include <iostream>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <unordered_set>
+#include <vector>
+
+#include "beam_hip.h"
+#include "graph_load.hpp"
+
+namespace {
+
+struct Cli {
+  std::string input, out_dir, profiling = "off";
+  std::string merge_mode = "batch-ea", backend = "hip", objective = "legacy", state = "legacy",
+              quotient_update = "incremental", index = "legacy", certification = "off", commit = "g0",
+              threshold = "reciprocal";
+  int top_k = 16, group_batch = 64, iterations = 20, print_offset = 1, hash_dims = 16, max_group = 512;
+  bool ea_threshold = false, write_output = false, help = false;
+  double error_bound = 0.0;
+  unsigned long long seed = 1;
+};
+
+void Usage() {
+  std::cout << "Usage:\n  beam --input <undirected-single-edge-list> [options]\n\n"
+            << "Runs BEAM's batch-ea-blocked merge with the MAGS-compatible objective,\n"
+            << "persistent quotient state, legacy candidate index, certification off and\n"
+            << "g0 commits on one MI355X (--scoring-backend hip).\n";
+}
+
+Cli Parse(int argc, char** argv) {  // main.cpp:327-423
+  Cli c;
+  auto need = [&](int& i) -> std::string {
+    if (i + 1 >= argc) throw std::runtime_error(std::string("Unknown or incomplete argument: ") + argv[i]);
+    return argv[++i];
+  };
+  for (int i = 1; i < argc; ++i) {
+    const std::string a = argv[i];
+    if (a == "--help" || a == "-h") c.help = true;
+    else if (a == "--input") c.input = need(i);
+    else if (a == "--merge-mode") c.merge_mode = need(i);
+    else if (a == "--scoring-backend") c.backend = need(i);
+    else if (a == "--cost-objective") c.objective = need(i);
+    else if (a == "--state-backend") c.state = need(i);
+    else if (a == "--quotient-update") c.quotient_update = need(i);
+    else if (a == "--candidate-index") c.index = need(i);
+    else if (a == "--certification") c.certification = need(i);
+    else if (a == "--commit-policy") c.commit = need(i);
+    else if (a == "--threshold-policy") c.threshold = need(i);
+    else if (a == "--profiling") c.profiling = need(i);
+    else if (a == "--top-k") c.top_k = std::stoi(need(i));
+    else if (a == "--group-batch-size") c.group_batch = std::stoi(need(i));
+    else if (a == "--iterations") c.iterations = std::stoi(need(i));
+    else if (a == "--print-offset") c.print_offset = std::stoi(need(i));
+    else if (a == "--divide-hash-dims") c.hash_dims = std::stoi(need(i));
+    else if (a == "--divide-max-group") c.max_group = std::stoi(need(i));
+    else if (a == "--error-bound") c.error_bound = std::stod(need(i));
+    else if (a == "--seed") c.seed = std::stoull(need(i));
+    else if (a == "--out") c.out_dir = need(i);
+    else if (a == "--ea-use-threshold") c.ea_threshold = true;
+    else if (a == "--write-output") c.write_output = true;
+    else throw std::runtime_error("Unknown or incomplete argument: " + a);
+  }
+  return c;
+}
+
+void Require(bool ok, const std::string& msg) {
+  if (!ok) throw std::runtime_error(msg);
+}
+
+void Check(int rc, const char* ctx) {
+  if (rc < 0) throw std::runtime_error(std::string(ctx) + " failed: " + beam_hip_last_error());
+}
+
+std::string Fixed(double v) {  // main.cpp:258-262
+  std::ostringstream o;
+  o.setf(std::ios::fixed);
+  o.precision(6);
+  o << v;
+  return o.str();
+}
+
+void Metric(const std::string& k, const std::string& v) { std::cout << "metric." << k << "=" << v << "\n"; }
+
+struct Encoding {
+  std::vector<std::vector<int>> supernodes;
+  std::vector<std::pair<int, int>> P, Cp, Cm;
+};
+
+// Sweg::Encode with the MAGS-compatible objective (sweg.cpp:3017-3163) over
+// the supernode lists of beam_hip_supernodes (already in Encode's order).
+Encoding Encode(const beam_host::Graph& g, const std::vector<int32_t>& offsets, const std::vector<int32_t>& members,
+                int count) {
+  Encoding r;
+  std::vector<int> node_to(static_cast<size_t>(g.n), -1);
+  r.supernodes.resize(static_cast<size_t>(count));
+  for (int s = 0; s < count; ++s) {
+    for (int k = offsets[static_cast<size_t>(s)]; k < offsets[static_cast<size_t>(s) + 1]; ++k) {
+      node_to[static_cast<size_t>(members[static_cast<size_t>(k)])] = s;
+      r.supernodes[static_cast<size_t>(s)].push_back(members[static_cast<size_t>(k)]);
+    }
+  }
+  struct Arc {
+    uint64_t key;
+    int u, v;
+  };
+  std::vector<Arc> arcs;
+  for (int u = 0; u < g.n; ++u) {
+    const int su = node_to[static_cast<size_t>(u)];
+    for (long long k = g.row_ptr[static_cast<size_t>(u)]; k < g.row_ptr[static_cast<size_t>(u) + 1]; ++k) {
+      const int v = g.col[static_cast<size_t>(k)];
+      const int sv = node_to[static_cast<size_t>(v)];
+      if (su > sv || (su == sv && u >= v)) continue;
+      arcs.push_back({(static_cast<uint64_t>(static_cast<uint32_t>(su)) << 32) | static_cast<uint32_t>(sv), u, v});
+    }
+  }
+  std::sort(arcs.begin(), arcs.end(), [](const Arc& a, const Arc& b) {
+    return a.key != b.key ? a.key < b.key : (a.u != b.u ? a.u < b.u : a.v < b.v);
+  });
+  size_t b = 0;
+  while (b < arcs.size()) {
+    size_t e = b + 1;
+    while (e < arcs.size() && arcs[e].key == arcs[b].key) ++e;
+    const int A = static_cast<int>(arcs[b].key >> 32), B = static_cast<int>(arcs[b].key & 0xffffffffU);
+    const long long cnt = static_cast<long long>(e - b);
+    const long long sa = static_cast<long long>(r.supernodes[static_cast<size_t>(A)].size());
+    const long long sb = static_cast<long long>(r.supernodes[static_cast<size_t>(B)].size());
+    const long long cap = A == B ? sa * (sa - 1) / 2 : sa * sb;
+    if (cnt <= 1 + cap - cnt) {
+      for (size_t i = b; i < e; ++i) r.Cp.emplace_back(arcs[i].u, arcs[i].v);
+    } else {
+      r.P.emplace_back(A, B);
+      std::unordered_set<uint64_t> present;
+      for (size_t i = b; i < e; ++i)
+        present.insert((static_cast<uint64_t>(static_cast<uint32_t>(arcs[i].u)) << 32) | static_cast<uint32_t>(arcs[i].v));
+      for (int u : r.supernodes[static_cast<size_t>(A)])
+        for (int v : r.supernodes[static_cast<size_t>(B)]) {
+          if (A == B && u >= v) continue;
+          if (!present.count((static_cast<uint64_t>(static_cast<uint32_t>(u)) << 32) | static_cast<uint32_t>(v)))
+            r.Cm.emplace_back(u, v);
+        }
+    }
+    b = e;
+  }
+  std::sort(r.P.begin(), r.P.end());
+  std::sort(r.Cp.begin(), r.Cp.end());
+  std::sort(r.Cm.begin(), r.Cm.end());
+  return r;
+}
+
+void WriteOutput(const std::string& dir, const Encoding& r) {  // sweg.cpp:3255-3287
+  std::filesystem::create_directories(dir);
+  std::ofstream g(dir + "/G.txt");
+  if (!g) throw std::runtime_error("Failed to open G.txt for writing.");
+  for (size_t s = 0; s < r.supernodes.size(); ++s) {
+    g << s;
+    for (int v : r.supernodes[s]) g << '\t' << v;
+    g << '\n';
+  }
+  auto edges = [&](const char* name, const std::vector<std::pair<int, int>>& list) {
+    std::ofstream o(dir + "/" + name);
+    if (!o) throw std::runtime_error(std::string("Failed to open output file: ") + name);
+    for (const auto& e : list) o << e.first << '\t' << e.second << '\n';
+  };
+  edges("P.txt", r.P);
+  edges("Cp.txt", r.Cp);
+  edges("Cm.txt", r.Cm);
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  try {
+    Cli c = Parse(argc, argv);
+    if (c.help || c.input.empty()) {
+      Usage();
+      return c.input.empty() && !c.help ? 1 : 0;
+    }
+    Require(!c.write_output || !c.out_dir.empty(), "--write-output requires --out <dir>.");
+    Require(c.backend == "hip", "this build implements --scoring-backend hip; the CPU path is the reference program");
+    Require(c.merge_mode == "batch-ea-blocked", "the HIP backend implements --merge-mode batch-ea-blocked");
+    Require(c.objective == "mags-compatible", "the HIP backend implements --cost-objective mags-compatible");
+    Require(c.state == "persistent", "the HIP backend implements --state-backend persistent");
+    Require(c.quotient_update == "incremental", "the HIP backend implements --quotient-update incremental");
+    Require(c.index == "legacy", "the HIP backend implements --candidate-index legacy");
+    Require(c.certification == "off", "the HIP backend implements --certification off");
+    Require(c.commit == "g0", "the HIP backend implements --commit-policy g0");
+    Require(c.threshold == "reciprocal" && c.ea_threshold,
+            "the HIP backend implements --ea-use-threshold with --threshold-policy reciprocal");
+    Require(c.error_bound == 0.0, "the HIP backend implements lossless --error-bound 0.0");
+    if (c.hash_dims <= 0) c.hash_dims = 16;
+    if (c.max_group <= 0) c.max_group = 512;
+
+    const beam_host::Graph g = beam_host::LoadEdgeList(c.input);
+    Require(g.arcs == 2 * g.input_edges_raw,
+            "Input graph invariant violated: graph_arcs must equal 2 * input_edges_raw.");
+    std::cout << "Loaded graph: n=" << g.n << " m=" << g.arcs << "\n"
+              << "Seed: " << c.seed << "\n"
+              << "Divide hash dims: " << c.hash_dims << "\nDivide max group: " << c.max_group << "\n"
+              << "Merge mode: batch-ea-blocked\nScoring backend: hip\nThreshold policy: reciprocal\n"
+              << "Cost objective: mags-compatible\nState backend: persistent\nQuotient update: incremental\n"
+              << "CUDA scoring build: " << beam_hip_build_mode() << "\n"
+              << (c.write_output ? "Output directory: " + c.out_dir : std::string("Output writing: disabled")) << "\n";
+
+    beam_hip_options o{c.top_k, c.group_batch, c.hash_dims, c.max_group, 1, 0, c.seed};
+    beam_hip_engine* e = nullptr;
+    Check(beam_hip_create(reinterpret_cast<const int64_t*>(g.row_ptr.data()), g.col.data(), g.n, g.arcs, &o, &e),
+          "beam_hip_create");
+    Check(beam_hip_run(e, c.iterations), "beam_hip_run");
+    double t[3];
+    beam_hip_timing(e, t);
+    std::vector<int32_t> labels(static_cast<size_t>(g.n));
+    Check(beam_hip_labels(e, labels.data()), "beam_hip_labels");
+    int64_t stats[BEAM_HIP_STAT_COUNT];
+    Check(beam_hip_stats(e, stats), "beam_hip_stats");
+    int64_t enc[6];
+    Check(beam_hip_encoding_counts(e, enc), "beam_hip_encoding_counts");
+    std::vector<int64_t> rounds(static_cast<size_t>(10 * std::max(c.iterations, 1)));
+    const int nr = beam_hip_round_stats(e, rounds.data(), c.iterations);
+
+    const auto enc_start = std::chrono::steady_clock::now();
+    double encode_ms = 0, output_ms = 0;
+    if (c.write_output) {
+      std::vector<int32_t> offsets(static_cast<size_t>(g.n) + 1), members(static_cast<size_t>(g.n));
+      const int count = beam_hip_supernodes(e, offsets.data(), members.data());
+      Check(count, "beam_hip_supernodes");
+      const Encoding r = Encode(g, offsets, members, count);
+      encode_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - enc_start).count();
+      if (static_cast<int64_t>(r.P.size()) != enc[0] || static_cast<int64_t>(r.Cp.size()) != enc[1] ||
+          static_cast<int64_t>(r.Cm.size()) != enc[2]) {
+        throw std::runtime_error("host encoding differs from the device block costs");
+      }
+      const auto w0 = std::chrono::steady_clock::now();
+      WriteOutput(c.out_dir, r);
+      output_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
+    }
+    const long long cost = enc[0] + enc[1] + enc[2];
+    Require(cost == enc[5], "MAGS-compatible payload cost differs from exact partition cost");
+    Metric("n", std::to_string(g.n));
+    Metric("input_edges_raw", std::to_string(g.input_edges_raw));
+    Metric("graph_arcs", std::to_string(g.arcs));
+    Metric("iterations", std::to_string(c.iterations));
+    Metric("seed", std::to_string(c.seed));
+    Metric("merge_mode", "batch-ea-blocked");
+    Metric("scoring_backend", "hip");
+    Metric("cost_objective", "mags-compatible");
+    Metric("partition_hash", std::to_string(beam_hip_partition_hash(labels.data(), g.n)));
+    Metric("exact_partition_cost", std::to_string(enc[5]));
+    Metric("supernode_count", std::to_string(enc[4]));
+    Metric("P", std::to_string(enc[0]));
+    Metric("Cp", std::to_string(enc[1]));
+    Metric("Cm", std::to_string(enc[2]));
+    Metric("num_superedges_loop", std::to_string(enc[3]));
+    Metric("num_superedges_nonloop", std::to_string(enc[0] - enc[3]));
+    Metric("encoding_cost_mags_x2", std::to_string(2 * cost));
+    Metric("cost_ratio_mags_compatible", Fixed(static_cast<double>(2 * cost) / (2.0 * static_cast<double>(g.input_edges_raw))));
+    Metric("runtime_run_ms", Fixed(t[0]));
+    Metric("runtime_divide_ms", Fixed(t[1]));
+    Metric("runtime_merge_ms", Fixed(t[2]));
+    Metric("runtime_encode_ms", Fixed(encode_ms));
+    Metric("runtime_output_ms", Fixed(output_ms));
+    static const char* names[] = {"divide_max_group_size", "divide_fallback_splits", "merge_candidate_pairs",
+                                  "merge_selected_pairs", "merge_total_local_gain", "merge_group_count",
+                                  "merge_group_max_size", "merge_raw_pair_count", "merge_candidate_pairs_after_prune",
+                                  "merge_exact_gain_calls", "merge_positive_gain_pairs", "merge_rejected_by_overlap",
+                                  "merge_rejected_by_threshold", "selected_merges_for_validation",
+                                  "accepted_merges_after_validation", "rejected_nonpositive", "validation_exact_calls",
+                                  "validation_exact_row_entry_work", "isolated_gain_sum", "hip_blocks", "hip_windows"};
+    for (int k = 0; k < BEAM_HIP_STAT_COUNT; ++k) Metric(names[k], std::to_string(stats[k]));
+    if (c.profiling == "rounds" || c.profiling == "summary") {
+      for (int i = 0; i < nr; ++i) {
+        const int64_t* r = &rounds[static_cast<size_t>(10 * i)];
+        std::cout << "round." << r[0] << " active_supernodes=" << r[1] << " group_count=" << r[2]
+                  << " candidate_proxy_pairs_examined=" << r[3] << " candidate_pairs_after_topk=" << r[4]
+                  << " exact_gain_calls=" << r[5] << " exact_gain_positive_count=" << r[6]
+                  << " above_threshold_count=" << r[7] << " matching_selected_count=" << r[8]
+                  << " actual_merge_count=" << r[9] << "\n";
+      }
+    }
+    beam_hip_destroy(e);
+  } catch (const std::exception& ex) {  // main.cpp:1489-1492
+    std::cerr << "Error: " << ex.what() << "\n";
+    return 1;
+  }
+  return 0;
+}

@@ -71,6 +71,16 @@ int beam_hip_iterate(beam_hip_engine* engine, int32_t iter);
  * src/sweg.cpp:2902-2918). */
 int beam_hip_labels(beam_hip_engine* engine, int32_t* labels);
 
+/* Supernodes of the current partition as the reference's Encode lists them
+ * (Sweg::Encode relabel stage, src/sweg.cpp:3019-3035, over Sweg::Recover_S,
+ * src/sweg.cpp:2966-2975): supernodes by ascending representative, members in
+ * linked-list order (the absorbing supernode's members, then each absorbed
+ * supernode's list in merge order, src/sweg.cpp:2999-3000).  offsets needs
+ * n + 1 entries and members n; supernode s holds
+ * members[offsets[s]..offsets[s+1]).  Returns the supernode count.  After
+ * beam_hip_upload_state, members uploaded together are listed by node id. */
+int beam_hip_supernodes(beam_hip_engine* engine, int32_t* offsets, int32_t* members);
+
 /* Summary counters, BEAM_HIP_STAT_COUNT entries in the order of the
  * BEAM_HIP_STAT_* indices (RuntimeStats, include/sweg.hpp:221-386); the last
  * two count the blocks merged and the speculative windows launched. */

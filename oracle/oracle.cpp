@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <filesystem>
 #include <fstream>
 #include <limits>
 #include <map>
@@ -906,6 +907,24 @@ Encoding Sweg::Encode() const {
   std::sort(r.Cp.begin(), r.Cp.end());
   std::sort(r.Cm.begin(), r.Cm.end());
   return r;
+}
+
+void Sweg::WriteOutput(const std::string& dir, const Encoding& e) {  // sweg.cpp:3255-3287
+  std::filesystem::create_directories(dir);
+  std::ofstream g(dir + "/G.txt");
+  if (!g) throw std::runtime_error("Failed to open G.txt for writing.");
+  for (size_t s = 0; s < e.supernodes.size(); ++s) {
+    g << s;
+    for (int v : e.supernodes[s]) g << '\t' << v;
+    g << '\n';
+  }
+  const std::pair<const char*, const std::vector<std::pair<int, int>>*> files[] = {
+      {"P.txt", &e.P}, {"Cp.txt", &e.Cp}, {"Cm.txt", &e.Cm}};
+  for (const auto& f : files) {
+    std::ofstream o(dir + "/" + f.first);
+    if (!o) throw std::runtime_error(std::string("Failed to open output file: ") + f.first);
+    for (const auto& p : *f.second) o << p.first << '\t' << p.second << '\n';
+  }
 }
 
 }  // namespace oracle

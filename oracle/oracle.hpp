@@ -134,6 +134,8 @@ class Sweg {
   uint64_t PartitionHash() const;  // sweg.cpp:2920-2928
   std::vector<int> Labels() const;  // sweg.cpp:2902-2918
   Encoding Encode() const;  // sweg.cpp:3017-3163 (mags-compatible branch)
+  // G.txt / P.txt / Cp.txt / Cm.txt of an encoding (sweg.cpp:3255-3287).
+  static void WriteOutput(const std::string& dir, const Encoding& e);
   int64_t ExactCurrentPartitionCost() const;
 
   const std::vector<std::pair<int, int>>& Groups() const { return groups_; }  // (start, length)

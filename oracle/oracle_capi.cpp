@@ -248,6 +248,14 @@ int orc_encode_counts(void* h, long long* out) {
   });
 }
 
+// Writes G.txt, P.txt, Cp.txt and Cm.txt of the current partition into dir.
+int orc_write_output(void* h, const char* dir) {
+  return Guard([&] {
+    oracle::Sweg::WriteOutput(dir, static_cast<Handle*>(h)->sweg->Encode());
+    return 0;
+  });
+}
+
 // Commit log of recorded blocks, flattened: for block i, counts[i] pairs.
 int orc_commit_log(void* h, int* counts, int* keeps, int* removes, int max_blocks, int max_pairs) {
   const auto& log = static_cast<Handle*>(h)->sweg->commit_log();

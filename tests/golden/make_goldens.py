@@ -10,6 +10,8 @@ the reference program with the benchmark flags and `--profiling rounds`, and
 stores the `metric.*` summary and the per-iteration profile columns as JSON.
 For the fa workload it also records per-pair exact gains from the reference's
 own QuotientGraph (oracle/ref_gain_harness.cpp) along a seeded merge script.
+`make_goldens.py outputs [fa am]` records digests of the files written by
+`--write-output` instead.
 """
 from __future__ import annotations
 
@@ -135,7 +137,29 @@ def gain_vectors(name: str = "fa", merges: int = 400, queries: int = 3000, rng_s
     return {"workload": name, "script": script, "answers": out.splitlines()}
 
 
+def output_digests(name: str) -> dict:
+    """SHA-256 and line count of G.txt, P.txt, Cp.txt, Cm.txt written by the
+    reference with --write-output (sweg.cpp:3255-3287)."""
+    import hashlib
+    import tempfile
+
+    with tempfile.TemporaryDirectory() as tmp:
+        subprocess.run([REF, "--input", graph_path(name), *FLAGS, "--seed", str(SEEDS[name]), "--write-output",
+                        "--out", tmp], check=True, capture_output=True, text=True)
+        files = {}
+        for fname in ["G.txt", "P.txt", "Cp.txt", "Cm.txt"]:
+            with open(os.path.join(tmp, fname), "rb") as f:
+                data = f.read()
+            files[fname] = {"sha256": hashlib.sha256(data).hexdigest(), "lines": data.count(b"\n")}
+    return {"workload": name, "seed": SEEDS[name], "flags": FLAGS, "files": files}
+
+
 def main(names: list[str]) -> None:
+    if names and names[0] == "outputs":
+        for name in names[1:] or ["fa", "am"]:
+            with open(os.path.join(OUT, f"{name}_output.json"), "w") as f:
+                json.dump(output_digests(name), f, indent=1)
+        return
     for name in names:
         summary = reference_summary(name)
         with open(os.path.join(OUT, f"{name}_summary.json"), "w") as f:

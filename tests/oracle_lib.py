@@ -55,6 +55,7 @@ def lib():
             "orc_export_quotient": (i32, [P, P, P, P, P, P, P]),
             "orc_encode_counts": (i32, [P, P]),
             "orc_commit_log": (i32, [P, P, P, P, i32, i32]),
+            "orc_write_output": (i32, [P, ctypes.c_char_p]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(L, name)
@@ -175,3 +176,6 @@ class Oracle:
         out = np.zeros(5, np.int64)
         _ok(lib().orc_encode_counts(self.h, _p(out)))
         return dict(zip(["P", "Cp", "Cm", "P_loops", "supernode_count"], (int(v) for v in out)))
+
+    def write_output(self, directory: str) -> None:
+        _ok(lib().orc_write_output(self.h, directory.encode()))

@@ -1,0 +1,80 @@
+"""The `beam` command line (beam_amd/host/beam_main.cpp): flag handling on the
+CPU, and on the GPU the metric lines and written encoding against the
+reference run recorded in tests/golden/fa_summary.json (src/main.cpp:577-956,
+src/sweg.cpp:3255-3287)."""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BEAM = os.path.join(ROOT, "beam_amd", "bin", "beam")
+FLAGS = [
+    "--merge-mode", "batch-ea-blocked", "--scoring-backend", "hip", "--cost-objective", "mags-compatible",
+    "--state-backend", "persistent", "--quotient-update", "incremental", "--candidate-index", "legacy",
+    "--certification", "off", "--commit-policy", "g0", "--top-k", "16", "--group-batch-size", "64",
+    "--ea-use-threshold", "--threshold-policy", "reciprocal", "--print-offset", "0", "--error-bound", "0.0",
+]
+
+
+def run(args, timeout=300):
+    return subprocess.run([BEAM, *args], capture_output=True, text=True, timeout=timeout)
+
+
+def test_help_and_unknown_flag():
+    assert run(["--help"]).returncode == 0
+    r = run(["--input", "x.txt", "--bogus"])
+    assert r.returncode == 1 and "Unknown or incomplete argument: --bogus" in r.stderr
+
+
+@pytest.mark.parametrize(
+    "change,message",
+    [
+        (("--scoring-backend", "cpu"), "--scoring-backend hip"),
+        (("--merge-mode", "batch-ea"), "batch-ea-blocked"),
+        (("--commit-policy", "g1"), "--commit-policy g0"),
+        (("--error-bound", "0.1"), "--error-bound 0.0"),
+    ],
+)
+def test_unsupported_configurations_are_refused(graph_path, change, message):
+    flags = list(FLAGS)
+    i = flags.index(change[0])
+    flags[i + 1] = change[1]
+    r = run(["--input", graph_path("tiny24"), *flags])
+    assert r.returncode == 1 and message in r.stderr
+
+
+def test_write_output_needs_out_dir(graph_path):
+    r = run(["--input", graph_path("tiny24"), *FLAGS, "--write-output"])
+    assert r.returncode == 1 and "--out" in r.stderr
+
+
+@pytest.mark.gpu
+def test_cli_matches_reference_run(graph_path, tmp_path):
+    with open(os.path.join(ROOT, "tests", "golden", "fa_summary.json")) as f:
+        golden = json.load(f)
+    out = tmp_path / "enc"
+    r = run(["--input", graph_path("fa"), *FLAGS, "--iterations", "20", "--seed", "1",
+             "--write-output", "--out", str(out)])
+    assert r.returncode == 0, r.stderr
+    m = dict(re.findall(r"^metric\.(\w+)=(.*)$", r.stdout, flags=re.M))
+    gm = golden["metrics"]
+    for key in ["n", "input_edges_raw", "graph_arcs", "partition_hash", "exact_partition_cost", "supernode_count",
+                "P", "Cp", "Cm", "num_superedges_loop", "num_superedges_nonloop", "encoding_cost_mags_x2",
+                "cost_ratio_mags_compatible", "divide_max_group_size", "merge_selected_pairs",
+                "merge_total_local_gain", "merge_group_count", "merge_raw_pair_count",
+                "merge_candidate_pairs_after_prune", "merge_exact_gain_calls", "merge_positive_gain_pairs",
+                "merge_rejected_by_overlap", "merge_rejected_by_threshold", "accepted_merges_after_validation",
+                "validation_exact_row_entry_work"]:
+        assert m[key] == gm[key], key
+    with open(os.path.join(ROOT, "tests", "golden", "fa_output.json")) as f:
+        files = json.load(f)["files"]
+    for fname, want in files.items():
+        data = (out / fname).read_bytes()
+        assert data.count(b"\n") == want["lines"], fname
+        assert hashlib.sha256(data).hexdigest() == want["sha256"], fname

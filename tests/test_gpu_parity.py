@@ -5,6 +5,7 @@ quotient rows (as multisets), Divide groups, partition hashes and counters.
 """
 from __future__ import annotations
 
+import hashlib
 import json
 import os
 
@@ -254,3 +255,14 @@ def test_end_to_end_matches_reference(graph_path, name, seed):
     assert len(rounds) == len(gold["rounds"])
     for got, want in zip(rounds, gold["rounds"]):
         assert got == want
+    # Supernode member lists in the reference's Recover_S order: G.txt as the
+    # reference writes it (sweg.cpp:3255-3270) has the same digest.
+    with open(os.path.join(GOLDEN, f"{name}_output.json")) as f:
+        want = json.load(f)["files"]["G.txt"]
+    offsets, members = engine.supernodes()
+    text = "".join(
+        str(s) + "".join("\t" + str(v) for v in members[offsets[s]:offsets[s + 1]]) + "\n"
+        for s in range(len(offsets) - 1)
+    ).encode()
+    assert text.count(b"\n") == want["lines"]
+    assert hashlib.sha256(text).hexdigest() == want["sha256"]

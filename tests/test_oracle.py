@@ -131,3 +131,25 @@ def test_bulk_merge_equals_sequential(graph_path):
             rems.append(max(a, b))
         assert o.merge_bulk(keeps, rems) == 0
         assert o.quotient_cost() == o.exact_partition_cost()
+
+
+def file_digests(directory) -> dict:
+    import hashlib
+
+    out = {}
+    for fname in ["G.txt", "P.txt", "Cp.txt", "Cm.txt"]:
+        with open(os.path.join(directory, fname), "rb") as f:
+            data = f.read()
+        out[fname] = {"sha256": hashlib.sha256(data).hexdigest(), "lines": data.count(b"\n")}
+    return out
+
+
+def test_oracle_output_files_match_reference(graph_path, tmp_path):
+    """G.txt lists members in Recover_S order (merge-history concatenation,
+    sweg.cpp:2966-2975, 2999-3000), so equal digests pin the commit order too."""
+    with open(os.path.join(GOLDEN, "fa_output.json")) as f:
+        gold = json.load(f)
+    o = Oracle(graph_path("fa"), seed=1)
+    o.run(20)
+    o.write_output(str(tmp_path))
+    assert file_digests(tmp_path) == gold["files"]
