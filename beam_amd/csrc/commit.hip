@@ -264,8 +264,11 @@ __device__ void SlotPrefix(const int32_t* sel_count, int nslots, uint32_t* off, 
   __syncthreads();
 }
 
+// Pair k of the flattened selections of nslots slots.  Slot s's pairs sit at
+// sel[2 * s * kMaxSelect] (window mode, gbase == nullptr) or at
+// sel[gbase[s]] (round mode, per group).
 __device__ __forceinline__ void WindowPair(const int32_t* sel, const uint32_t* off, int nslots, uint32_t k,
-                                           int* slot, int* keep, int* remove) {
+                                           const int32_t* gbase, int* slot, int* keep, int* remove) {
   int lo = 0, hi = nslots;
   while (hi - lo > 1) {
     const int mid = (lo + hi) >> 1;
@@ -275,144 +278,12 @@ __device__ __forceinline__ void WindowPair(const int32_t* sel, const uint32_t* o
   while (lo + 1 < nslots && off[lo + 1] <= k) ++lo;
   const uint32_t local = k - off[lo];
   *slot = lo;
-  *keep = sel[2 * (lo * kMaxSelect + static_cast<int>(local))];
-  *remove = sel[2 * (lo * kMaxSelect + static_cast<int>(local)) + 1];
+  const long long base = gbase != nullptr ? static_cast<long long>(gbase[lo]) : 2LL * lo * kMaxSelect;
+  *keep = sel[base + 2 * static_cast<long long>(local)];
+  *remove = sel[base + 2 * static_cast<long long>(local) + 1];
 }
 
 }  // namespace
-
-// Speculative window resolution.  Every group of the window was scored on the
-// same state; block j of the window may use its selection only if no member
-// of its groups is adjacent to a keep or remove selected by blocks 0..j-1 of
-// the window (those are exactly the rows and neighbour sizes the merges
-// change).  All workgroups record, per node, the first window block whose
-// merges touch it; the last workgroup to finish finds the first block with a
-// touched member, marks and queues the merges of the blocks before it, and
-// adds their statistics to the iteration counters.
-__global__ __launch_bounds__(kThreads) void ResolveKernel(ResolveParams p) {
-  __shared__ uint32_t off[kMaxWindowSlots + 1];
-  __shared__ uint32_t wave_tot[kThreads / kWave + 1];
-  __shared__ int s_last, s_j;
-  __shared__ unsigned long long s_acc[kSlotStats];
-  __shared__ unsigned long long s_max;
-  const int tid = static_cast<int>(threadIdx.x);
-  SlotPrefix(p.sel_count, p.nslots, off, wave_tot);
-  const uint32_t total = off[p.nslots];
-  for (uint32_t item = blockIdx.x; item < total; item += gridDim.x) {
-    int slot, keep, remove;
-    WindowPair(p.sel, off, p.nslots, item, &slot, &keep, &remove);
-    const unsigned long long val =
-        (static_cast<unsigned long long>(p.epoch) << 16) | static_cast<unsigned long long>(0xffff - slot / p.block_size);
-    const int lk = p.q.row_len[keep], lr = p.q.row_len[remove];
-    const Entry* rk = p.q.pool + p.q.row_off[keep];
-    const Entry* rr = p.q.pool + p.q.row_off[remove];
-    for (int e = tid; e < lk + lr; e += kThreads) {
-      const int x = e < lk ? rk[e].nbr : rr[e - lk].nbr;
-      atomicMax(&p.touch[x], val);
-    }
-  }
-  __threadfence();
-  __syncthreads();
-  if (tid == 0) s_last = atomicAdd(p.done, 1U) == gridDim.x - 1 ? 1 : 0;
-  __syncthreads();
-  if (!s_last) return;
-  __threadfence();
-  const int nblocks = (p.nslots + p.block_size - 1) / p.block_size;
-  if (tid == 0) s_j = nblocks;
-  if (tid < kSlotStats) s_acc[tid] = 0;
-  if (tid == 0) s_max = 0;
-  __syncthreads();
-  for (int s = tid; s < p.nslots; s += kThreads) {
-    const int rel = s / p.block_size;
-    if (rel == 0) continue;
-    const int gidx = p.first_group + s;
-    const int g = p.glen[gidx];
-    if (g < 2) continue;
-    const int start = p.gstart[gidx];
-    for (int i = 0; i < g; ++i) {
-      const unsigned long long t = p.touch[p.order[start + i]];
-      if (static_cast<long long>(t >> 16) == p.epoch && 0xffff - static_cast<int>(t & 0xffff) < rel) {
-        atomicMin(&s_j, rel);
-        break;
-      }
-    }
-  }
-  __syncthreads();
-  const int j = s_j;
-  const int cslots = min(p.nslots, j * p.block_size);
-  const uint32_t npairs = off[cslots];
-  // Merge marks of the committed pairs (Update_S / QuotientGraph::Merge run
-  // on them, sweg.cpp:2745-2747).
-  for (uint32_t item = tid; item < npairs; item += kThreads) {
-    int slot, keep, remove;
-    WindowPair(p.sel, off, cslots, item, &slot, &keep, &remove);
-    p.mark_dest[remove] = static_cast<int32_t>(p.epoch);
-    p.dest[remove] = keep;
-    p.mark_keep[keep] = static_cast<int32_t>(p.epoch);
-    p.parent[remove] = keep;
-    p.merge_seq[remove] = static_cast<int32_t>(p.epoch);
-  }
-  // Neighbours of every committed remove are queued for re-pointing.
-  for (uint32_t item = tid; item < npairs; item += kThreads) {
-    int slot, keep, remove;
-    WindowPair(p.sel, off, cslots, item, &slot, &keep, &remove);
-    const Entry* rr = p.q.pool + p.q.row_off[remove];
-    for (int e = 0; e < p.q.row_len[remove]; ++e) {
-      const int x = rr[e].nbr;
-      if (x == keep) continue;
-      if (atomicExch(&p.mark_row[x], static_cast<int32_t>(p.epoch)) != static_cast<int32_t>(p.epoch)) {
-        p.aff[atomicAdd(p.naff, 1)] = x;
-      }
-    }
-  }
-  // Statistics of the committed groups.
-  unsigned long long acc[kSlotStats] = {};
-  unsigned long long mx = 0;
-  for (int s = tid; s < cslots; s += kThreads) {
-    const long long* st = p.slot_stats + static_cast<long long>(s) * kSlotStats;
-    const long long g = st[kStatGroup];
-    if (g < 2) continue;
-    acc[kStatGroup] += 1;
-    acc[kStatCand] += static_cast<unsigned long long>(st[kStatCand]);
-    acc[kStatProxy] += static_cast<unsigned long long>(st[kStatProxy]);
-    acc[kStatPositive] += static_cast<unsigned long long>(st[kStatPositive]);
-    acc[kStatPass] += static_cast<unsigned long long>(st[kStatPass]);
-    acc[kStatOverlap] += static_cast<unsigned long long>(st[kStatOverlap]);
-    acc[kStatChosen] += static_cast<unsigned long long>(st[kStatChosen]);
-    acc[kStatGain] += static_cast<unsigned long long>(st[kStatGain]);
-    acc[kStatWork] += static_cast<unsigned long long>(st[kStatWork]);
-    acc[kStatEntries] += static_cast<unsigned long long>(st[kStatEntries]);
-    acc[kSlotStats - 1] += static_cast<unsigned long long>(g * (g - 1) / 2);
-    acc[kSlotStats - 2] += static_cast<unsigned long long>(g);
-    mx = max(mx, static_cast<unsigned long long>(g));
-  }
-  for (int k = 0; k < kSlotStats; ++k) {
-    if (acc[k]) atomicAdd(&s_acc[k], acc[k]);
-  }
-  if (mx) atomicMax(&s_max, mx);
-  __syncthreads();
-  if (tid == 0) {
-    Counters* c = p.ctr;
-    c->group_count += s_acc[kStatGroup];
-    c->group_max_size = max(c->group_max_size, s_max);
-    c->raw_pair_count += s_acc[kSlotStats - 1];
-    c->candidate_pairs_after_prune += s_acc[kStatCand];
-    c->proxy_pairs_examined += s_acc[kStatProxy];
-    c->exact_gain_calls += s_acc[kStatCand];
-    c->positive_gain_count += s_acc[kStatPositive];
-    c->above_threshold += s_acc[kStatPass];
-    c->rejected_by_threshold += s_acc[kStatCand] - s_acc[kStatPass];
-    c->rejected_by_overlap += s_acc[kStatOverlap];
-    c->selected += s_acc[kStatChosen];
-    c->total_local_gain += static_cast<long long>(s_acc[kStatGain]);
-    c->validation_row_entry_work += s_acc[kStatWork];
-    c->member_rows += s_acc[kSlotStats - 2];
-    c->member_entries += s_acc[kStatEntries];
-    *p.commit_slots = cslots;
-    *p.commit_blocks = j;
-    *p.done = 0;
-  }
-}
 
 __global__ __launch_bounds__(kThreads) void CommitKernel(CommitParams p) {
   __shared__ CommitShared sh;
@@ -423,19 +294,195 @@ __global__ __launch_bounds__(kThreads) void CommitKernel(CommitParams p) {
     *p.naff_next = 0;
   }
   const int nslots = *p.nslots;
-  SlotPrefix(p.sel_count, nslots, sh.slot_off, wave_tot);
+  const int first = p.first_group != nullptr ? *p.first_group : 0;
+  const int32_t* gbase = p.first_group != nullptr ? p.gstart + first : nullptr;
+  SlotPrefix(p.sel_count + first, nslots, sh.slot_off, wave_tot);
   const uint32_t npairs = sh.slot_off[nslots];
   const uint32_t naff = static_cast<uint32_t>(*p.naff);
   for (uint32_t item = blockIdx.x; item < npairs + naff; item += gridDim.x) {
     if (item < npairs) {
       int slot, keep, remove;
-      WindowPair(p.sel, sh.slot_off, nslots, item, &slot, &keep, &remove);
+      WindowPair(p.sel, sh.slot_off, nslots, item, gbase, &slot, &keep, &remove);
       if (!UnionPair(p, sh, keep, remove)) return;
     } else {
       const int x = p.aff[item - npairs];
       if (p.mark_dest[x] == p.epoch || p.mark_keep[x] == p.epoch) continue;
       if (!RewriteRow(p, sh, x)) return;
     }
+  }
+}
+
+// ---------------------------------------------------------------- rounds
+//
+// A round's groups were all scored on the state the round starts from (the
+// queue held every group of the horizon that was new or touched since its
+// last scoring).  Block b of the horizon may commit in this round iff no
+// member of its groups is adjacent to a keep or remove selected by blocks
+// before b -- exactly the rows and neighbour sizes those merges change --
+// so committing the leading such blocks together equals committing them one
+// after another (sweg.cpp:953-1145).
+
+// Records, per node, the first horizon block whose selected pairs touch it.
+__global__ __launch_bounds__(kThreads) void TouchRound(RoundParams p) {
+  const int cursor = p.cur->cursor, end = p.cur->end;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    p.cur->j = end - cursor;
+    p.next->ntodo = 0;
+    p.next->steal = 0;
+    *p.union_top = 0;
+  }
+  const int first = cursor * p.block_size;
+  const int last = min(p.ngroups, end * p.block_size);
+  const int lane = LaneId();
+  const int waves = static_cast<int>(gridDim.x) * (kThreads / kWave);
+  for (int gi = first + static_cast<int>(blockIdx.x) * (kThreads / kWave) + WaveId(); gi < last; gi += waves) {
+    const int cnt = p.glen[gi] >= 2 ? p.sel_count[gi] : 0;
+    if (cnt == 0) continue;
+    const unsigned long long val = (static_cast<unsigned long long>(p.epoch) << 16) |
+                                   static_cast<unsigned long long>(0xffff - (gi / p.block_size - cursor));
+    const int32_t* sp = p.sel + p.gstart[gi];
+    for (int k = 0; k < cnt; ++k) {
+      const int keep = sp[2 * k], remove = sp[2 * k + 1];
+      const int lk = p.q.row_len[keep], lr = p.q.row_len[remove];
+      const Entry* rk = p.q.pool + p.q.row_off[keep];
+      const Entry* rr = p.q.pool + p.q.row_off[remove];
+      for (int e = lane; e < lk + lr; e += kWave) {
+        const int x = e < lk ? rk[e].nbr : rr[e - lk].nbr;
+        atomicMax(&p.touch[x], val);
+      }
+    }
+  }
+}
+
+// j = the first horizon block (after the cursor's) with a member touched by
+// an earlier block of the horizon; blocks cursor .. cursor + j - 1 commit.
+__global__ __launch_bounds__(256) void FindCommitBlocks(RoundParams p) {
+  const int cursor = p.cur->cursor, end = p.cur->end;
+  const int first = (cursor + 1) * p.block_size;
+  const int last = min(p.ngroups, end * p.block_size);
+  if (first >= last) return;
+  const int pos0 = p.gstart[first];
+  const int pos1 = p.gstart[last - 1] + p.glen[last - 1];
+  const int stride = static_cast<int>(gridDim.x * blockDim.x);
+  for (int pos = pos0 + static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x); pos < pos1; pos += stride) {
+    const int x = p.order[pos];
+    const int gi = p.node_group[x];
+    if (p.glen[gi] < 2) continue;
+    const int rel = gi / p.block_size - cursor;
+    const unsigned long long t = p.touch[x];
+    if (static_cast<int>(t >> 16) == p.epoch && 0xffff - static_cast<int>(t & 0xffff) < rel) {
+      atomicMin(&p.cur->j, rel);
+    }
+  }
+}
+
+// Marks the merges of the committed blocks (Update_S / QuotientGraph::Merge
+// run on them, sweg.cpp:2745-2747), queues the neighbour rows of every remove
+// for re-pointing, queues for rescoring every uncommitted group of the next
+// horizon that a merge touched and every group entering the horizon, and
+// adds the committed groups' statistics to the iteration counters.
+__global__ __launch_bounds__(kThreads) void MarkRound(RoundParams p) {
+  __shared__ unsigned long long s_acc[kSlotStats];
+  __shared__ unsigned long long s_max;
+  const int tid = static_cast<int>(threadIdx.x);
+  const int bs = p.block_size;
+  const int cursor = p.cur->cursor, end = p.cur->end, j = p.cur->j;
+  const int ncur = cursor + j;
+  const int nend = min(p.nblocks, ncur + p.horizon);
+  const int first = cursor * bs;
+  const int clast = min(p.ngroups, ncur * bs);
+  if (blockIdx.x == 0 && tid == 0) {
+    p.next->cursor = ncur;
+    p.next->end = nend;
+    p.commit_out[0] = clast - first;
+    p.commit_out[1] = j;
+    p.commit_out[2] = first;
+  }
+  if (tid < kSlotStats) s_acc[tid] = 0;
+  if (tid == 0) s_max = 0;
+  __syncthreads();
+  const int lane = LaneId();
+  const int waves = static_cast<int>(gridDim.x) * (kThreads / kWave);
+  const int wave = static_cast<int>(blockIdx.x) * (kThreads / kWave) + WaveId();
+  const int stamp = p.epoch + 1;
+  auto queue = [&](int gx) {
+    if (atomicExch(&p.gmark[gx], stamp) != stamp) p.todo_next[atomicAdd(&p.next->ntodo, 1)] = gx;
+  };
+  unsigned long long acc[kSlotStats] = {};
+  unsigned long long mx = 0;
+  for (int gi = first + wave; gi < clast; gi += waves) {
+    const int g = p.glen[gi];
+    if (g < 2) continue;
+    if (lane == 0) {
+      const long long* st = p.slot_stats + static_cast<long long>(gi) * kSlotStats;
+      acc[kStatGroup] += 1;
+      acc[kStatCand] += static_cast<unsigned long long>(st[kStatCand]);
+      acc[kStatProxy] += static_cast<unsigned long long>(st[kStatProxy]);
+      acc[kStatPositive] += static_cast<unsigned long long>(st[kStatPositive]);
+      acc[kStatPass] += static_cast<unsigned long long>(st[kStatPass]);
+      acc[kStatOverlap] += static_cast<unsigned long long>(st[kStatOverlap]);
+      acc[kStatChosen] += static_cast<unsigned long long>(st[kStatChosen]);
+      acc[kStatGain] += static_cast<unsigned long long>(st[kStatGain]);
+      acc[kStatWork] += static_cast<unsigned long long>(st[kStatWork]);
+      acc[kStatEntries] += static_cast<unsigned long long>(st[kStatEntries]);
+      acc[kSlotStats - 1] += static_cast<unsigned long long>(g) * static_cast<unsigned long long>(g - 1) / 2;
+      acc[kSlotStats - 2] += static_cast<unsigned long long>(g);
+      mx = max(mx, static_cast<unsigned long long>(g));
+    }
+    const int cnt = p.sel_count[gi];
+    const int32_t* sp = p.sel + p.gstart[gi];
+    for (int k = 0; k < cnt; ++k) {
+      const int keep = sp[2 * k], remove = sp[2 * k + 1];
+      if (lane == 0) {
+        p.mark_dest[remove] = p.epoch;
+        p.dest[remove] = keep;
+        p.mark_keep[keep] = p.epoch;
+        p.parent[remove] = keep;
+        p.merge_seq[remove] = p.epoch;
+      }
+      const int lk = p.q.row_len[keep], lr = p.q.row_len[remove];
+      const Entry* rk = p.q.pool + p.q.row_off[keep];
+      const Entry* rr = p.q.pool + p.q.row_off[remove];
+      for (int e = lane; e < lk + lr; e += kWave) {
+        const int x = e < lk ? rk[e].nbr : rr[e - lk].nbr;
+        const int gx = p.node_group[x];
+        const int bx = gx / bs;
+        if (bx >= ncur && bx < nend && p.glen[gx] >= 2) queue(gx);
+        if (e >= lk && x != keep && atomicExch(&p.mark_row[x], p.epoch) != p.epoch) {
+          p.aff[atomicAdd(p.naff, 1)] = x;
+        }
+      }
+    }
+  }
+  // Groups entering the horizon.
+  const int nlast = min(p.ngroups, nend * bs);
+  for (int gi = end * bs + wave * kWave + lane; gi < nlast; gi += waves * kWave) {
+    if (p.glen[gi] >= 2) queue(gi);
+  }
+  if (lane == 0) {
+    for (int k = 0; k < kSlotStats; ++k) {
+      if (acc[k]) atomicAdd(&s_acc[k], acc[k]);
+    }
+    if (mx) atomicMax(&s_max, mx);
+  }
+  __syncthreads();
+  if (tid == 0 && s_acc[kStatGroup] != 0) {
+    Counters* c = p.ctr;
+    atomicAdd(&c->group_count, s_acc[kStatGroup]);
+    atomicMax(&c->group_max_size, s_max);
+    atomicAdd(&c->raw_pair_count, s_acc[kSlotStats - 1]);
+    atomicAdd(&c->candidate_pairs_after_prune, s_acc[kStatCand]);
+    atomicAdd(&c->proxy_pairs_examined, s_acc[kStatProxy]);
+    atomicAdd(&c->exact_gain_calls, s_acc[kStatCand]);
+    atomicAdd(&c->positive_gain_count, s_acc[kStatPositive]);
+    atomicAdd(&c->above_threshold, s_acc[kStatPass]);
+    atomicAdd(&c->rejected_by_threshold, s_acc[kStatCand] - s_acc[kStatPass]);
+    atomicAdd(&c->rejected_by_overlap, s_acc[kStatOverlap]);
+    atomicAdd(&c->selected, s_acc[kStatChosen]);
+    atomicAdd(reinterpret_cast<unsigned long long*>(&c->total_local_gain), s_acc[kStatGain]);
+    atomicAdd(&c->validation_row_entry_work, s_acc[kStatWork]);
+    atomicAdd(&c->member_rows, s_acc[kSlotStats - 2]);
+    atomicAdd(&c->member_entries, s_acc[kStatEntries]);
   }
 }
 

@@ -21,7 +21,6 @@ thread_local std::string g_error;
 
 constexpr int kMaxIterations = 4096;
 constexpr int kCommitGrid = 512;
-constexpr int kResolveGrid = 256;
 
 template <typename T>
 T* DevAlloc(size_t count) {
@@ -231,6 +230,34 @@ __global__ void GroupLens(int G, int A, int32_t* gstart, int32_t* glen) {
   glen[j] = end - gstart[j];
 }
 
+// Group of every active representative for this iteration's rounds.
+__global__ void NodeGroups(int G, const int32_t* gstart, const int32_t* glen, const int32_t* order, int32_t* node_group) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= G) return;
+  for (int i = 0; i < glen[g]; ++i) node_group[order[gstart[g] + i]] = g;
+}
+
+// State of an iteration's first round: empty queue, horizon [0, end), the
+// round's neighbour queue and group scratch cleared.
+__global__ void InitRound(RoundState* rs, int end, int32_t* naff, unsigned long long* group_top) {
+  if (threadIdx.x != 0) return;
+  rs->cursor = 0;
+  rs->end = end;
+  rs->j = 0;
+  rs->ntodo = 0;
+  rs->steal = 0;
+  *naff = 0;
+  *group_top = 0;
+}
+
+// Queues the groups [0, last) with at least two members for scoring.
+__global__ void QueueGroups(int last, const int32_t* glen, int32_t* gmark, int stamp, int32_t* todo, RoundState* rs) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= last || glen[g] < 2) return;
+  gmark[g] = stamp;
+  todo[atomicAdd(&rs->ntodo, 1)] = g;
+}
+
 __global__ void FillU32(uint32_t* p, int n, uint32_t v) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -402,9 +429,16 @@ struct Engine {
   int32_t* sel_count = nullptr;
   long long* slot_stats = nullptr;
   unsigned long long* touch = nullptr;
-  int32_t* commit_out = nullptr;  // [0] committed slots, [1] committed blocks
-  unsigned int* done = nullptr;
-  int window_blocks = 32;
+  int32_t* commit_out = nullptr;  // [0] committed slots, [1] committed blocks, [2] first group
+  int32_t commit_host[2] = {0, 0};
+  int32_t* node_group = nullptr;  // rounds: group of every active rep
+  int32_t* gmark = nullptr;       // rounds: epoch of the queue holding each group
+  int32_t* todo[2] = {nullptr, nullptr};
+  RoundState* rs = nullptr;       // [2], alternating between rounds
+  int horizon_blocks = 64;
+  int score_grid = 256;
+  long long rounds_run = 0;
+  double blocks_per_round = 8.0;
   long long windows_run = 0;
   int32_t* mark_dest = nullptr;
   int32_t* dest = nullptr;
@@ -450,10 +484,12 @@ struct Engine {
   void CheckDeviceError(const char* where);
   int Divide(int iter);
   void MergeBlocks(int iter, int ngroups, double theta, Counters* c);
-  int LaunchWindow(int first_group, int nslots, int ngroups, int block_size, double theta, Counters* c,
-                   const int32_t* order_arr, const int32_t* gstart_arr, const int32_t* glen_arr, int32_t* dbg_pairs,
-                   long long* dbg_gain, int32_t* dbg_count, int dbg_cap, bool commit);
-  void Commit(Counters* c);
+  void LaunchRound(int G, int nblocks, int block_size, double theta, Counters* c);
+  void LaunchWindow(int first_group, int nslots, int ngroups, double theta, Counters* c,
+                    const int32_t* order_arr, const int32_t* gstart_arr, const int32_t* glen_arr, int32_t* dbg_pairs,
+                    long long* dbg_gain, int32_t* dbg_count, int dbg_cap);
+  void Commit(Counters* c, bool round = false);
+  void CommitSlots(int nslots, Counters* c);
   void EnsurePoolRoom();
   void Iterate(int iter);
   void Reset();
@@ -469,7 +505,7 @@ Engine::~Engine() {
                   open_rep, mh,       recs[0],  recs[1],   flag,     excl,       rid,       run_start, run_open,
                   run_seg,  elem_open, new_idx, gflag,     gstart,   glen,       d_total,   d_maxlen, sel,
                   sel_count, mark_dest, dest,   mark_keep, mark_row, aff,        naff,      scratch_a, scratch_u,
-                  slot_stats, touch,  commit_out, done,
+                  slot_stats, touch,  commit_out, node_group, gmark,     todo[0],   todo[1], rs,
                   tops,     ctr,      err,      enc,        trace_buf};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -539,12 +575,26 @@ void Engine::Create(const int64_t* h_row_ptr, const int32_t* h_col, int n_, long
   d_total = DevAlloc<uint32_t>(4);
   d_maxlen = DevAlloc<unsigned long long>(1);
 
-  sel = DevAlloc<int32_t>(static_cast<size_t>(kMaxWindowSlots) * kMaxSelect * 2);
-  sel_count = DevAlloc<int32_t>(kMaxWindowSlots);
-  slot_stats = DevAlloc<long long>(static_cast<size_t>(kMaxWindowSlots) * kSlotStats);
+  // Window mode indexes selections by slot, rounds by group (sel by the
+  // group's start position).
+  const size_t nsel = std::max<size_t>(static_cast<size_t>(kMaxWindowSlots) * kMaxSelect * 2, static_cast<size_t>(n) + 1);
+  const size_t nslot = std::max<size_t>(kMaxWindowSlots, static_cast<size_t>(n) + 1);
+  sel = DevAlloc<int32_t>(nsel);
+  sel_count = DevAlloc<int32_t>(nslot);
+  slot_stats = DevAlloc<long long>(nslot * kSlotStats);
   touch = DevAlloc<unsigned long long>(n);
-  commit_out = DevAlloc<int32_t>(2);
-  done = DevAlloc<unsigned int>(1);
+  commit_out = DevAlloc<int32_t>(4);
+  node_group = DevAlloc<int32_t>(n);
+  gmark = DevAlloc<int32_t>(n);
+  todo[0] = DevAlloc<int32_t>(n);
+  todo[1] = DevAlloc<int32_t>(n);
+  rs = DevAlloc<RoundState>(2);
+  {
+    int cus = 0, per_cu = 0;
+    HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, opt.device));
+    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, MergeGroupsKernel, 512, 0));
+    score_grid = std::max(1, cus) * std::max(1, per_cu);
+  }
   mark_dest = DevAlloc<int32_t>(n);
   dest = DevAlloc<int32_t>(n);
   mark_keep = DevAlloc<int32_t>(n);
@@ -578,6 +628,7 @@ void Engine::Reset() {
   HIP_CHECK(hipMemcpyAsync(pool_top, &top, sizeof(top), hipMemcpyHostToDevice, stream));
   HIP_CHECK(hipMemsetAsync(mark_dest, 0, sizeof(int32_t) * n, stream));
   HIP_CHECK(hipMemsetAsync(merge_seq, 0, sizeof(int32_t) * n, stream));
+  HIP_CHECK(hipMemsetAsync(gmark, 0, sizeof(int32_t) * n, stream));
   HIP_CHECK(hipMemsetAsync(mark_keep, 0, sizeof(int32_t) * n, stream));
   HIP_CHECK(hipMemsetAsync(mark_row, 0, sizeof(int32_t) * n, stream));
   HIP_CHECK(hipMemsetAsync(naff, 0, 2 * sizeof(int32_t), stream));
@@ -585,10 +636,11 @@ void Engine::Reset() {
   HIP_CHECK(hipMemsetAsync(ctr, 0, sizeof(Counters) * (kMaxIterations + 1), stream));
   HIP_CHECK(hipMemsetAsync(err, 0, sizeof(int), stream));
   HIP_CHECK(hipMemsetAsync(touch, 0, sizeof(unsigned long long) * n, stream));
-  HIP_CHECK(hipMemsetAsync(done, 0, sizeof(unsigned int), stream));
   epoch = 0;
   windows_run = 0;
   blocks_run = 0;
+  rounds_run = 0;
+  blocks_per_round = 8.0;
   divide_max_group = 0;
   divide_fallback_splits = 0;
   round_rows.clear();
@@ -721,13 +773,11 @@ int Engine::Divide(int iter) {
   return G;
 }
 
-// Scores the groups [first_group, first_group + nslots) speculatively on the
-// current state (one workgroup per group), then -- with commit -- resolves how
-// many leading blocks of block_size groups can be committed together and
-// commits them.  Returns the number of committed blocks (0 without commit).
-int Engine::LaunchWindow(int first_group, int nslots, int ngroups, int block_size, double theta, Counters* c,
-                         const int32_t* order_arr, const int32_t* gstart_arr, const int32_t* glen_arr,
-                         int32_t* dbg_pairs, long long* dbg_gain, int32_t* dbg_count, int dbg_cap, bool commit) {
+// Scores the groups [first_group, first_group + nslots) on the current state,
+// one workgroup per group, into slots 0 .. nslots - 1 (no commit).
+void Engine::LaunchWindow(int first_group, int nslots, int ngroups, double theta, Counters* c,
+                          const int32_t* order_arr, const int32_t* gstart_arr, const int32_t* glen_arr,
+                          int32_t* dbg_pairs, long long* dbg_gain, int32_t* dbg_count, int dbg_cap) {
   ++epoch;
   MergeParams mp{};
   mp.q = QuotientView{pool[pool_idx], row_off, row_len, size, internal};
@@ -766,26 +816,72 @@ int Engine::LaunchWindow(int first_group, int nslots, int ngroups, int block_siz
   MergeGroupsKernel<<<nslots, 512, 0, stream>>>(mp);
   if (profile) HIP_CHECK(hipEventRecord(ev_pool[ev_used++], stream));
   ++windows_run;
-  if (!commit) {
-    // No commit zeroes the group scratch top for the next launch.
-    HIP_CHECK(hipMemsetAsync(&tops[0], 0, sizeof(unsigned long long), stream));
-    return 0;
-  }
+  // The group scratch top is zeroed for the next launch.
+  HIP_CHECK(hipMemsetAsync(&tops[0], 0, sizeof(unsigned long long), stream));
+}
 
-  ResolveParams rp{};
+// One merge round (RoundState): score the queued groups on the current state,
+// find the leading blocks of the horizon that commit together, mark them and
+// queue the next round's groups, then merge.  Every kernel reads the round's
+// position from device memory, so rounds are enqueued without host waits; a
+// round after the iteration's last block does nothing.
+void Engine::LaunchRound(int G, int nblocks, int block_size, double theta, Counters* c) {
+  ++epoch;
+  if (epoch >= (1LL << 30)) throw std::runtime_error("round counter overflow");
+  const int e32 = static_cast<int>(epoch);
+  RoundState* cur = rs + (epoch & 1);
+  RoundState* nxt = rs + ((epoch + 1) & 1);
+  MergeParams mp{};
+  mp.q = QuotientView{pool[pool_idx], row_off, row_len, size, internal};
+  mp.order = order;
+  mp.gstart = gstart;
+  mp.glen = glen;
+  mp.ngroups = G;
+  mp.top_k = opt.top_k;
+  mp.theta = theta;
+  mp.sel = sel;
+  mp.sel_count = sel_count;
+  mp.slot_stats = slot_stats;
+  mp.scratch = scratch_a;
+  mp.scratch_cap = scratch_a_cap;
+  mp.scratch_top = &tops[0];
+  mp.ctr = c;
+  mp.err = err;
+  mp.trace = (trace_buf != nullptr && tracing_now) ? trace_buf : nullptr;
+  mp.round = cur;
+  mp.todo = todo[epoch & 1];
+  if (profile) {
+    if (ev_used + 2 > ev_pool.size()) {
+      for (int k = 0; k < 256; ++k) {
+        hipEvent_t ev;
+        HIP_CHECK(hipEventCreate(&ev));
+        ev_pool.push_back(ev);
+      }
+    }
+    HIP_CHECK(hipEventRecord(ev_pool[ev_used++], stream));
+  }
+  MergeGroupsKernel<<<score_grid, 512, 0, stream>>>(mp);
+  if (profile) HIP_CHECK(hipEventRecord(ev_pool[ev_used++], stream));
+
+  RoundParams rp{};
   rp.q = mp.q;
-  rp.order = order_arr;
-  rp.gstart = gstart_arr;
-  rp.glen = glen_arr;
-  rp.ngroups = ngroups;
-  rp.first_group = first_group;
-  rp.nslots = nslots;
+  rp.order = order;
+  rp.gstart = gstart;
+  rp.glen = glen;
+  rp.node_group = node_group;
+  rp.ngroups = G;
+  rp.nblocks = nblocks;
   rp.block_size = block_size;
-  rp.epoch = epoch;
+  rp.horizon = horizon_blocks;
+  rp.epoch = e32;
+  rp.cur = cur;
+  rp.next = nxt;
   rp.sel = sel;
   rp.sel_count = sel_count;
   rp.slot_stats = slot_stats;
   rp.touch = touch;
+  rp.gmark = gmark;
+  rp.todo_next = todo[(epoch + 1) & 1];
   rp.mark_dest = mark_dest;
   rp.dest = dest;
   rp.mark_keep = mark_keep;
@@ -794,21 +890,37 @@ int Engine::LaunchWindow(int first_group, int nslots, int ngroups, int block_siz
   rp.mark_row = mark_row;
   rp.aff = aff;
   rp.naff = &naff[epoch & 1];
-  rp.commit_slots = &commit_out[0];
-  rp.commit_blocks = &commit_out[1];
-  rp.done = done;
+  rp.commit_out = commit_out;
+  rp.union_top = &tops[1];
   rp.ctr = c;
   rp.err = err;
-  ResolveKernel<<<kResolveGrid, 512, 0, stream>>>(rp);
-  Commit(c);
-  int32_t blocks = 0;
-  HIP_CHECK(hipMemcpyAsync(&blocks, &commit_out[1], sizeof(blocks), hipMemcpyDeviceToHost, stream));
-  Sync();
-  blocks_run += blocks;
-  return blocks;
+  const int slots = horizon_blocks * block_size;
+  const int wave_grid = (slots + 7) / 8;  // one wave per group, 8 waves per workgroup
+  TouchRound<<<wave_grid, 512, 0, stream>>>(rp);
+  FindCommitBlocks<<<256, 256, 0, stream>>>(rp);
+  MarkRound<<<wave_grid, 512, 0, stream>>>(rp);
+  Commit(c, true);
+  ++windows_run;
+  ++rounds_run;
 }
 
-void Engine::Commit(Counters* c) {
+// Commits the pairs held in slots 0 .. nslots - 1 of the window-mode
+// selection arrays as one block.
+void Engine::CommitSlots(int nslots, Counters* c) {
+  ++epoch;
+  HIP_CHECK(hipMemsetAsync(&naff[epoch & 1], 0, sizeof(int32_t), stream));
+  HIP_CHECK(hipMemsetAsync(&tops[1], 0, sizeof(unsigned long long), stream));
+  MarkCommit<<<nslots, 256, 0, stream>>>(sel, sel_count, nslots, static_cast<int>(epoch), mark_dest, dest, mark_keep,
+                                         parent, merge_seq, QuotientView{pool[pool_idx], row_off, row_len, size, internal},
+                                         mark_row, aff, &naff[epoch & 1]);
+  commit_host[0] = nslots;
+  commit_host[1] = 1;
+  HIP_CHECK(hipMemcpyAsync(commit_out, commit_host, 2 * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+  Commit(c);
+  Sync();
+}
+
+void Engine::Commit(Counters* c, bool round) {
   CommitParams cp{};
   cp.pool = pool[pool_idx];
   cp.row_off = row_off;
@@ -820,6 +932,8 @@ void Engine::Commit(Counters* c) {
   cp.sel = sel;
   cp.sel_count = sel_count;
   cp.nslots = &commit_out[0];
+  cp.first_group = round ? &commit_out[2] : nullptr;
+  cp.gstart = gstart;
   cp.epoch = static_cast<int>(epoch);
   cp.mark_dest = mark_dest;
   cp.dest = dest;
@@ -870,18 +984,33 @@ void Engine::Iterate(int iter) {
   }
   const int bs = opt.group_batch_size;
   const int nblocks = (G + bs - 1) / bs;
-  const int wb = std::max(1, std::min(window_blocks, kMaxWindowSlots / bs));
-  int cursor = 0;
-  while (cursor < nblocks) {
-    const int first = cursor * bs;
-    const int nslots = std::min(wb * bs, G - first);
-    const int committed = LaunchWindow(first, nslots, G, bs, theta, c, order, gstart, glen, nullptr, nullptr, nullptr, 0,
-                                       true);
-    if (committed <= 0) {
+  if (nblocks > 0) {
+    // Rounds: the horizon spans at most kMaxWindowSlots groups (the commit
+    // kernel's slot prefix lives in LDS) and 0xffff blocks (touch encoding).
+    horizon_blocks = std::max(1, std::min(64, kMaxWindowSlots / bs));
+    NodeGroups<<<Blocks(G, 256), 256, 0, stream>>>(G, gstart, glen, order, node_group);
+    FillU32<<<Blocks(G, 256), 256, 0, stream>>>(reinterpret_cast<uint32_t*>(sel_count), G, 0U);
+    const int first_end = std::min(nblocks, horizon_blocks);
+    RoundState* first_rs = rs + ((epoch + 1) & 1);
+    InitRound<<<1, 64, 0, stream>>>(first_rs, first_end, &naff[(epoch + 1) & 1], &tops[0]);
+    const int qlast = std::min(G, first_end * bs);
+    QueueGroups<<<Blocks(qlast, 256), 256, 0, stream>>>(qlast, glen, gmark, static_cast<int>(epoch + 1),
+                                                         todo[(epoch + 1) & 1], first_rs);
+    int cursor = 0;
+    while (cursor < nblocks) {
+      const double est = static_cast<double>(nblocks - cursor) / std::max(1.0, blocks_per_round);
+      const int batch = static_cast<int>(std::min(256.0, std::max(2.0, std::ceil(est))));
+      for (int b = 0; b < batch; ++b) LaunchRound(G, nblocks, bs, theta, c);
+      int32_t h_cursor = 0;
+      HIP_CHECK(hipMemcpyAsync(&h_cursor, &rs[(epoch + 1) & 1].cursor, sizeof(h_cursor), hipMemcpyDeviceToHost, stream));
+      Sync();
       CheckDeviceError("merge");
-      throw std::runtime_error("window resolution committed no block");
+      if (h_cursor <= cursor) throw std::runtime_error("merge rounds made no progress");
+      const int progressed = h_cursor - cursor;
+      blocks_per_round = 0.5 * blocks_per_round + 0.5 * static_cast<double>(progressed) / batch;
+      cursor = h_cursor;
     }
-    cursor += committed;
+    blocks_run += nblocks;
   }
   Sync();
   tracing_now = false;
@@ -1294,8 +1423,8 @@ int beam_hip_merge_groups(beam_hip_engine* engine, const int32_t* reps, int32_t 
     HIP_CHECK(hipMemcpyAsync(dln, lengths, sizeof(int32_t) * ngroups, hipMemcpyHostToDevice, e.stream));
     HIP_CHECK(hipMemsetAsync(dcount, 0, sizeof(int32_t) * ngroups, e.stream));
     beam::Counters* c = e.ctr;  // slot 0 collects counters of per-stage calls
-    e.LaunchWindow(0, ngroups, ngroups, ngroups, theta, c, dreps, dst, dln, dpairs, dgain, dcount, cand_cap,
-                   commit != 0);
+    e.LaunchWindow(0, ngroups, ngroups, theta, c, dreps, dst, dln, dpairs, dgain, dcount, cand_cap);
+    if (commit != 0) e.CommitSlots(ngroups, c);
     HIP_CHECK(hipMemcpyAsync(cand_count, dcount, sizeof(int32_t) * ngroups, hipMemcpyDeviceToHost, e.stream));
     HIP_CHECK(hipMemcpyAsync(cand_pairs, dpairs, sizeof(int32_t) * ngroups * cand_cap * 2, hipMemcpyDeviceToHost, e.stream));
     HIP_CHECK(hipMemcpyAsync(cand_gain, dgain, sizeof(long long) * ngroups * cand_cap * 2, hipMemcpyDeviceToHost, e.stream));
@@ -1344,14 +1473,7 @@ int beam_hip_commit(beam_hip_engine* engine, const int32_t* keeps, const int32_t
     }
     HIP_CHECK(hipMemcpyAsync(e.sel, hsel.data(), hsel.size() * sizeof(int32_t), hipMemcpyHostToDevice, e.stream));
     HIP_CHECK(hipMemcpyAsync(e.sel_count, hcount.data(), hcount.size() * sizeof(int32_t), hipMemcpyHostToDevice, e.stream));
-    ++e.epoch;
-    beam::MarkCommit<<<nslots, 256, 0, e.stream>>>(
-        e.sel, e.sel_count, nslots, static_cast<int>(e.epoch), e.mark_dest, e.dest, e.mark_keep, e.parent,
-        e.merge_seq, beam::QuotientView{e.pool[e.pool_idx], e.row_off, e.row_len, e.size, e.internal}, e.mark_row, e.aff,
-        &e.naff[e.epoch & 1]);
-    const int32_t counts[2] = {nslots, 1};
-    HIP_CHECK(hipMemcpyAsync(e.commit_out, counts, sizeof(counts), hipMemcpyHostToDevice, e.stream));
-    e.Commit(e.ctr);
+    e.CommitSlots(nslots, e.ctr);
     e.Sync();
     e.CheckDeviceError("commit");
     return 0;

@@ -16,6 +16,14 @@
 //   label     int32[n] node -> representative (S_, sweg.hpp:690)
 //   csr       original graph (row_ptr int64[n+1], col_idx int32[arcs]) for
 //             Divide's min-hash shingles (sweg.cpp:754-764)
+//
+// Per iteration (indexed by group g of Divide's order):
+//   sel       int32   selected (keep, remove) pairs of group g at
+//                     sel[gstart[g] ..): a group of g members selects at most
+//                     g / 2 disjoint pairs, so the groups' ranges never overlap
+//   sel_count int32, slot_stats int64[kSlotStats] per group
+//   node_group int32[n] group of every active representative
+//   gmark     int32   epoch of the round whose queue holds the group
 #pragma once
 
 #include "common.hpp"
@@ -24,7 +32,7 @@ namespace beam {
 
 constexpr int kMaxSelect = 256;          // selected pairs per group (<= 512 / 2)
 constexpr int kMaxSlotsPerBlock = 1024;  // groups per block (--group-batch-size)
-constexpr int kMaxWindowSlots = 4096;    // groups scored speculatively per launch
+constexpr int kMaxWindowSlots = 4096;    // groups per scoring launch / round horizon
 
 // Per-slot (group) statistics written by the group kernel.
 enum : int {
@@ -47,6 +55,20 @@ struct QuotientView {
   const int32_t* row_len;
   const int32_t* size;
   const int32_t* internal;
+};
+
+// Merge rounds.  A round scores the queued groups of the horizon
+// [cursor, end) of blocks on the current state, finds the leading blocks
+// whose groups no earlier block of the horizon touches, commits them and
+// queues the groups their merges touched (and the blocks that enter the
+// horizon) for the next round.  Two states alternate between rounds.
+struct RoundState {
+  int cursor;  // first uncommitted block of the iteration
+  int end;     // horizon end (exclusive, blocks)
+  int j;       // blocks this round commits
+  int ntodo;   // groups queued for scoring in this round
+  int steal;   // scoring work counter
+  int pad[3];
 };
 
 struct MergeParams {
@@ -76,35 +98,42 @@ struct MergeParams {
   long long* dbg_gain;
   int32_t* dbg_count;
   int dbg_cap;
+  // Round mode (round != nullptr): workgroups take groups from todo[0..ntodo)
+  // and write sel / sel_count / slot_stats per group instead of per slot.
+  RoundState* round;
+  const int32_t* todo;
 };
 
-// Resolution of a speculative window: which leading blocks can be committed
-// together (ResolveKernel), then their merges marked and queued.
-struct ResolveParams {
+// One merge round's bookkeeping kernels (see RoundState).
+struct RoundParams {
   QuotientView q;
   const int32_t* order;
   const int32_t* gstart;
   const int32_t* glen;
+  const int32_t* node_group;
   int ngroups;
-  int first_group;        // first group of the window
-  int nslots;             // groups in the window
-  int block_size;         // groups per block
-  long long epoch;        // unique per window; tags touches and merge marks
+  int nblocks;
+  int block_size;
+  int horizon;              // blocks
+  int epoch;                // unique per round
+  RoundState* cur;
+  RoundState* next;
   const int32_t* sel;
   const int32_t* sel_count;
   const long long* slot_stats;
   unsigned long long* touch;  // (epoch << 16) | (0xffff - first touching block)
+  int32_t* gmark;
+  int32_t* todo_next;
   int32_t* mark_dest;
   int32_t* dest;
   int32_t* mark_keep;
   int32_t* parent;
-  int32_t* merge_seq;     // window epoch of each committed remove
+  int32_t* merge_seq;
   int32_t* mark_row;
   int32_t* aff;
   int32_t* naff;
-  int32_t* commit_slots;  // out: slots committed (whole blocks)
-  int32_t* commit_blocks; // out: blocks committed
-  unsigned int* done;     // workgroups finished (reset by the last one)
+  int32_t* commit_out;        // [0] groups committed, [1] blocks, [2] first group
+  unsigned long long* union_top;  // the commit's scratch top, zeroed by TouchRound
   Counters* ctr;
   int* err;
 };
@@ -120,6 +149,9 @@ struct CommitParams {
   const int32_t* sel;
   const int32_t* sel_count;
   const int32_t* nslots;       // slots to commit (device)
+  // Round mode: groups first_group[0] + s; group s's pairs at sel[gstart[.]].
+  const int32_t* first_group;
+  const int32_t* gstart;
   int epoch;
   const int32_t* mark_dest;
   const int32_t* dest;
@@ -138,7 +170,9 @@ struct CommitParams {
 };
 
 __global__ void MergeGroupsKernel(MergeParams p);
-__global__ void ResolveKernel(ResolveParams p);
 __global__ void CommitKernel(CommitParams p);
+__global__ void TouchRound(RoundParams p);
+__global__ void FindCommitBlocks(RoundParams p);
+__global__ void MarkRound(RoundParams p);
 
 }  // namespace beam

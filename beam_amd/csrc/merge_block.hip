@@ -288,24 +288,20 @@ __device__ void ScoreSegment(const GroupArrays& A, const int32_t* m_rep, const i
   }
 }
 
-}  // namespace
-
-__global__ __launch_bounds__(kThreads) void MergeGroupsKernel(MergeParams p) {
+// Candidate generation, exact scoring and selection for group gidx; results
+// go to sel_out (pairs), sel_count[slot], slot_stats[slot].  Every early exit
+// is taken by the whole workgroup.
+__device__ void ScoreGroup(const MergeParams& p, int gidx, int slot, int32_t* sel_out) {
   long long* trace = nullptr;
   const long long t_start = static_cast<long long>(wall_clock64());
   auto stamp = [&](int k) {
     if (trace != nullptr && threadIdx.x == 0) trace[k] = static_cast<long long>(wall_clock64());
   };
-  const int slot = static_cast<int>(blockIdx.x);
-  const int gidx = p.first_group + slot;
   const int tid = static_cast<int>(threadIdx.x);
   if (tid == 0) {
     p.sel_count[slot] = 0;
     p.slot_stats[static_cast<long long>(slot) * kSlotStats + kStatGroup] = 0;
-    if (slot == 0 && p.reset_top != nullptr) *p.reset_top = 0;
-    if (slot == 0 && p.reset_queue != nullptr) *p.reset_queue = 0;
   }
-  if (gidx >= p.ngroups) return;
   const int g = p.glen[gidx];
   if (g < 2) return;
   if (g > kMaxGroup) {
@@ -766,8 +762,8 @@ __global__ __launch_bounds__(kThreads) void MergeGroupsKernel(MergeParams p) {
       }
       s_used[a] = s_used[b] = 1;
       const int keep = m_rep[a], remove = m_rep[b];
-      p.sel[2 * (slot * kMaxSelect + chosen)] = keep;
-      p.sel[2 * (slot * kMaxSelect + chosen) + 1] = remove;
+      sel_out[2 * chosen] = keep;
+      sel_out[2 * chosen + 1] = remove;
       work += static_cast<unsigned long long>(m_len[a] + m_len[b]);
       total += A.cand_gain[px];
       ++chosen;
@@ -800,6 +796,38 @@ __global__ __launch_bounds__(kThreads) void MergeGroupsKernel(MergeParams p) {
   }
   __syncthreads();
   stamp(13);
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(kThreads) void MergeGroupsKernel(MergeParams p) {
+  if (p.round == nullptr) {
+    // Window mode: workgroup s scores group first_group + s into slot s.
+    const int slot = static_cast<int>(blockIdx.x);
+    if (threadIdx.x == 0) {
+      p.sel_count[slot] = 0;
+      p.slot_stats[static_cast<long long>(slot) * kSlotStats + kStatGroup] = 0;
+      if (slot == 0 && p.reset_top != nullptr) *p.reset_top = 0;
+      if (slot == 0 && p.reset_queue != nullptr) *p.reset_queue = 0;
+    }
+    const int gidx = p.first_group + slot;
+    if (gidx >= p.ngroups) return;
+    ScoreGroup(p, gidx, slot, p.sel + 2LL * slot * kMaxSelect);
+    return;
+  }
+  // Round mode: workgroups take queued groups until the queue is drained;
+  // results are kept per group.
+  __shared__ int s_item;
+  const int ntodo = p.round->ntodo;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = atomicAdd(&p.round->steal, 1);
+    __syncthreads();
+    const int item = s_item;
+    __syncthreads();
+    if (item >= ntodo) break;
+    const int gidx = p.todo[item];
+    ScoreGroup(p, gidx, gidx, p.sel + p.gstart[gidx]);
+  }
 }
 
 }  // namespace beam
