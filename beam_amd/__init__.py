@@ -213,10 +213,11 @@ class Engine:
         _check(self.lib.beam_hip_profile(self.h, 1 if enable else 0))
 
     def kernel_profile(self) -> dict:
-        out = np.zeros(5, dtype=np.float64)
+        out = np.zeros(8, dtype=np.float64)
         _check(self.lib.beam_hip_kernel_profile(self.h, _ptr(out)))
         return {"group_kernel_ms": float(out[0]), "group_kernel_launches": int(out[1]),
-                "member_rows": int(out[2]), "member_entries": int(out[3]), "pairs_scored": int(out[4])}
+                "member_rows": int(out[2]), "member_entries": int(out[3]), "pairs_scored": int(out[4]),
+                "rounds": int(out[5]), "rounds_committing": int(out[6]), "groups_scored": int(out[7])}
 
     def trace(self, it: int) -> None:
         """Records the group kernel's phase stamps during iteration `it`."""

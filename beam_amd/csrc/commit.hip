@@ -330,6 +330,7 @@ __global__ __launch_bounds__(kThreads) void TouchRound(RoundParams p) {
     p.next->ntodo = 0;
     p.next->steal = 0;
     *p.union_top = 0;
+    p.round_ctr[1] += static_cast<unsigned long long>(p.cur->ntodo);
   }
   const int first = cursor * p.block_size;
   const int last = min(p.ngroups, end * p.block_size);
@@ -378,12 +379,9 @@ __global__ __launch_bounds__(256) void FindCommitBlocks(RoundParams p) {
 
 // Marks the merges of the committed blocks (Update_S / QuotientGraph::Merge
 // run on them, sweg.cpp:2745-2747), queues the neighbour rows of every remove
-// for re-pointing, queues for rescoring every uncommitted group of the next
-// horizon that a merge touched and every group entering the horizon, and
-// adds the committed groups' statistics to the iteration counters.
+// for re-pointing, and queues for rescoring every uncommitted group of the
+// next horizon that a merge touched and every group entering the horizon.
 __global__ __launch_bounds__(kThreads) void MarkRound(RoundParams p) {
-  __shared__ unsigned long long s_acc[kSlotStats];
-  __shared__ unsigned long long s_max;
   const int tid = static_cast<int>(threadIdx.x);
   const int bs = p.block_size;
   const int cursor = p.cur->cursor, end = p.cur->end, j = p.cur->j;
@@ -397,10 +395,8 @@ __global__ __launch_bounds__(kThreads) void MarkRound(RoundParams p) {
     p.commit_out[0] = clast - first;
     p.commit_out[1] = j;
     p.commit_out[2] = first;
+    if (j > 0) p.round_ctr[0] += 1;
   }
-  if (tid < kSlotStats) s_acc[tid] = 0;
-  if (tid == 0) s_max = 0;
-  __syncthreads();
   const int lane = LaneId();
   const int waves = static_cast<int>(gridDim.x) * (kThreads / kWave);
   const int wave = static_cast<int>(blockIdx.x) * (kThreads / kWave) + WaveId();
@@ -408,27 +404,8 @@ __global__ __launch_bounds__(kThreads) void MarkRound(RoundParams p) {
   auto queue = [&](int gx) {
     if (atomicExch(&p.gmark[gx], stamp) != stamp) p.todo_next[atomicAdd(&p.next->ntodo, 1)] = gx;
   };
-  unsigned long long acc[kSlotStats] = {};
-  unsigned long long mx = 0;
   for (int gi = first + wave; gi < clast; gi += waves) {
-    const int g = p.glen[gi];
-    if (g < 2) continue;
-    if (lane == 0) {
-      const long long* st = p.slot_stats + static_cast<long long>(gi) * kSlotStats;
-      acc[kStatGroup] += 1;
-      acc[kStatCand] += static_cast<unsigned long long>(st[kStatCand]);
-      acc[kStatProxy] += static_cast<unsigned long long>(st[kStatProxy]);
-      acc[kStatPositive] += static_cast<unsigned long long>(st[kStatPositive]);
-      acc[kStatPass] += static_cast<unsigned long long>(st[kStatPass]);
-      acc[kStatOverlap] += static_cast<unsigned long long>(st[kStatOverlap]);
-      acc[kStatChosen] += static_cast<unsigned long long>(st[kStatChosen]);
-      acc[kStatGain] += static_cast<unsigned long long>(st[kStatGain]);
-      acc[kStatWork] += static_cast<unsigned long long>(st[kStatWork]);
-      acc[kStatEntries] += static_cast<unsigned long long>(st[kStatEntries]);
-      acc[kSlotStats - 1] += static_cast<unsigned long long>(g) * static_cast<unsigned long long>(g - 1) / 2;
-      acc[kSlotStats - 2] += static_cast<unsigned long long>(g);
-      mx = max(mx, static_cast<unsigned long long>(g));
-    }
+    if (p.glen[gi] < 2) continue;
     const int cnt = p.sel_count[gi];
     const int32_t* sp = p.sel + p.gstart[gi];
     for (int k = 0; k < cnt; ++k) {
@@ -459,15 +436,45 @@ __global__ __launch_bounds__(kThreads) void MarkRound(RoundParams p) {
   for (int gi = end * bs + wave * kWave + lane; gi < nlast; gi += waves * kWave) {
     if (p.glen[gi] >= 2) queue(gi);
   }
-  if (lane == 0) {
-    for (int k = 0; k < kSlotStats; ++k) {
-      if (acc[k]) atomicAdd(&s_acc[k], acc[k]);
-    }
-    if (mx) atomicMax(&s_max, mx);
+}
+
+// Iteration counters from the final statistics of every group: each group is
+// committed exactly once, after its last scoring, so its slot holds the
+// figures of the scoring that was committed.
+__global__ __launch_bounds__(kThreads) void SumGroupStats(const long long* slot_stats, const int32_t* glen, int ngroups,
+                                                         Counters* c) {
+  __shared__ unsigned long long s_acc[kSlotStats];
+  __shared__ unsigned long long s_max;
+  const int tid = static_cast<int>(threadIdx.x);
+  if (tid < kSlotStats) s_acc[tid] = 0;
+  if (tid == 0) s_max = 0;
+  __syncthreads();
+  unsigned long long acc[kSlotStats] = {};
+  unsigned long long mx = 0;
+  for (int gi = static_cast<int>(blockIdx.x) * kThreads + tid; gi < ngroups; gi += static_cast<int>(gridDim.x) * kThreads) {
+    const int g = glen[gi];
+    if (g < 2) continue;
+    const long long* st = slot_stats + static_cast<long long>(gi) * kSlotStats;
+    acc[kStatGroup] += 1;
+    acc[kStatCand] += static_cast<unsigned long long>(st[kStatCand]);
+    acc[kStatProxy] += static_cast<unsigned long long>(st[kStatProxy]);
+    acc[kStatPositive] += static_cast<unsigned long long>(st[kStatPositive]);
+    acc[kStatPass] += static_cast<unsigned long long>(st[kStatPass]);
+    acc[kStatOverlap] += static_cast<unsigned long long>(st[kStatOverlap]);
+    acc[kStatChosen] += static_cast<unsigned long long>(st[kStatChosen]);
+    acc[kStatGain] += static_cast<unsigned long long>(st[kStatGain]);
+    acc[kStatWork] += static_cast<unsigned long long>(st[kStatWork]);
+    acc[kStatEntries] += static_cast<unsigned long long>(st[kStatEntries]);
+    acc[kSlotStats - 1] += static_cast<unsigned long long>(g) * static_cast<unsigned long long>(g - 1) / 2;
+    acc[kSlotStats - 2] += static_cast<unsigned long long>(g);
+    mx = max(mx, static_cast<unsigned long long>(g));
   }
+  for (int k = 0; k < kSlotStats; ++k) {
+    if (acc[k]) atomicAdd(&s_acc[k], acc[k]);
+  }
+  if (mx) atomicMax(&s_max, mx);
   __syncthreads();
   if (tid == 0 && s_acc[kStatGroup] != 0) {
-    Counters* c = p.ctr;
     atomicAdd(&c->group_count, s_acc[kStatGroup]);
     atomicMax(&c->group_max_size, s_max);
     atomicAdd(&c->raw_pair_count, s_acc[kSlotStats - 1]);

@@ -435,10 +435,11 @@ struct Engine {
   int32_t* gmark = nullptr;       // rounds: epoch of the queue holding each group
   int32_t* todo[2] = {nullptr, nullptr};
   RoundState* rs = nullptr;       // [2], alternating between rounds
+  unsigned long long* round_ctr = nullptr;  // [2] productive rounds, groups scored
   int horizon_blocks = 64;
   int score_grid = 256;
   long long rounds_run = 0;
-  double blocks_per_round = 8.0;
+  double blocks_per_round = 32.0;
   long long windows_run = 0;
   int32_t* mark_dest = nullptr;
   int32_t* dest = nullptr;
@@ -505,7 +506,7 @@ Engine::~Engine() {
                   open_rep, mh,       recs[0],  recs[1],   flag,     excl,       rid,       run_start, run_open,
                   run_seg,  elem_open, new_idx, gflag,     gstart,   glen,       d_total,   d_maxlen, sel,
                   sel_count, mark_dest, dest,   mark_keep, mark_row, aff,        naff,      scratch_a, scratch_u,
-                  slot_stats, touch,  commit_out, node_group, gmark,     todo[0],   todo[1], rs,
+                  slot_stats, touch,  commit_out, node_group, gmark,     todo[0],   todo[1], rs, round_ctr,
                   tops,     ctr,      err,      enc,        trace_buf};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -589,6 +590,7 @@ void Engine::Create(const int64_t* h_row_ptr, const int32_t* h_col, int n_, long
   todo[0] = DevAlloc<int32_t>(n);
   todo[1] = DevAlloc<int32_t>(n);
   rs = DevAlloc<RoundState>(2);
+  round_ctr = DevAlloc<unsigned long long>(2);
   {
     int cus = 0, per_cu = 0;
     HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, opt.device));
@@ -629,6 +631,7 @@ void Engine::Reset() {
   HIP_CHECK(hipMemsetAsync(mark_dest, 0, sizeof(int32_t) * n, stream));
   HIP_CHECK(hipMemsetAsync(merge_seq, 0, sizeof(int32_t) * n, stream));
   HIP_CHECK(hipMemsetAsync(gmark, 0, sizeof(int32_t) * n, stream));
+  HIP_CHECK(hipMemsetAsync(round_ctr, 0, 2 * sizeof(unsigned long long), stream));
   HIP_CHECK(hipMemsetAsync(mark_keep, 0, sizeof(int32_t) * n, stream));
   HIP_CHECK(hipMemsetAsync(mark_row, 0, sizeof(int32_t) * n, stream));
   HIP_CHECK(hipMemsetAsync(naff, 0, 2 * sizeof(int32_t), stream));
@@ -640,7 +643,7 @@ void Engine::Reset() {
   windows_run = 0;
   blocks_run = 0;
   rounds_run = 0;
-  blocks_per_round = 8.0;
+  blocks_per_round = 32.0;
   divide_max_group = 0;
   divide_fallback_splits = 0;
   round_rows.clear();
@@ -892,6 +895,7 @@ void Engine::LaunchRound(int G, int nblocks, int block_size, double theta, Count
   rp.naff = &naff[epoch & 1];
   rp.commit_out = commit_out;
   rp.union_top = &tops[1];
+  rp.round_ctr = round_ctr;
   rp.ctr = c;
   rp.err = err;
   const int slots = horizon_blocks * block_size;
@@ -997,20 +1001,34 @@ void Engine::Iterate(int iter) {
     QueueGroups<<<Blocks(qlast, 256), 256, 0, stream>>>(qlast, glen, gmark, static_cast<int>(epoch + 1),
                                                          todo[(epoch + 1) & 1], first_rs);
     int cursor = 0;
+    unsigned long long productive = 0;
+    HIP_CHECK(hipMemcpyAsync(&productive, round_ctr, sizeof(productive), hipMemcpyDeviceToHost, stream));
     while (cursor < nblocks) {
+      // Rounds left at the recent rate (blocks per committing round); a
+      // little less is enqueued so that few rounds run idle past the last
+      // block (each costs a few launches; a short batch one more host wait).
       const double est = static_cast<double>(nblocks - cursor) / std::max(1.0, blocks_per_round);
-      const int batch = static_cast<int>(std::min(256.0, std::max(2.0, std::ceil(est))));
+      const int batch = static_cast<int>(std::min(48.0, std::max(1.0, std::ceil(0.75 * est))));
       for (int b = 0; b < batch; ++b) LaunchRound(G, nblocks, bs, theta, c);
-      int32_t h_cursor = 0;
-      HIP_CHECK(hipMemcpyAsync(&h_cursor, &rs[(epoch + 1) & 1].cursor, sizeof(h_cursor), hipMemcpyDeviceToHost, stream));
+      struct {
+        int32_t cursor;
+        unsigned long long productive;
+      } h{};
+      HIP_CHECK(hipMemcpyAsync(&h.cursor, &rs[(epoch + 1) & 1].cursor, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+      HIP_CHECK(hipMemcpyAsync(&h.productive, round_ctr, sizeof(h.productive), hipMemcpyDeviceToHost, stream));
       Sync();
       CheckDeviceError("merge");
-      if (h_cursor <= cursor) throw std::runtime_error("merge rounds made no progress");
-      const int progressed = h_cursor - cursor;
-      blocks_per_round = 0.5 * blocks_per_round + 0.5 * static_cast<double>(progressed) / batch;
-      cursor = h_cursor;
+      if (h.cursor <= cursor) throw std::runtime_error("merge rounds made no progress");
+      const unsigned long long used = h.productive - productive;
+      if (used > 0) {
+        const double rate = static_cast<double>(h.cursor - cursor) / static_cast<double>(used);
+        blocks_per_round = 0.5 * blocks_per_round + 0.5 * rate;
+      }
+      productive = h.productive;
+      cursor = h.cursor;
     }
     blocks_run += nblocks;
+    SumGroupStats<<<std::max(1, std::min(256, Blocks(G, 512))), 512, 0, stream>>>(slot_stats, glen, G, c);
   }
   Sync();
   tracing_now = false;
@@ -1261,13 +1279,21 @@ int beam_hip_profile(beam_hip_engine* engine, int32_t enable) {
 }
 
 int beam_hip_kernel_profile(beam_hip_engine* engine, double* out) {
-  const auto& e = engine->e;
-  out[0] = e.group_kernel_ms;
-  out[1] = static_cast<double>(e.group_kernel_launches);
-  out[2] = static_cast<double>(e.member_rows);
-  out[3] = static_cast<double>(e.member_entries);
-  out[4] = static_cast<double>(e.pairs_scored);
-  return 0;
+  return Guard([&] {
+    auto& e = engine->e;
+    unsigned long long rc[2] = {0, 0};
+    HIP_CHECK(hipMemcpyAsync(rc, e.round_ctr, sizeof(rc), hipMemcpyDeviceToHost, e.stream));
+    e.Sync();
+    out[0] = e.group_kernel_ms;
+    out[1] = static_cast<double>(e.group_kernel_launches);
+    out[2] = static_cast<double>(e.member_rows);
+    out[3] = static_cast<double>(e.member_entries);
+    out[4] = static_cast<double>(e.pairs_scored);
+    out[5] = static_cast<double>(e.rounds_run);
+    out[6] = static_cast<double>(rc[0]);
+    out[7] = static_cast<double>(rc[1]);
+    return 0;
+  });
 }
 
 uint64_t beam_hip_partition_hash(const int32_t* labels, int32_t n) {

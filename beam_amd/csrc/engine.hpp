@@ -134,6 +134,7 @@ struct RoundParams {
   int32_t* naff;
   int32_t* commit_out;        // [0] groups committed, [1] blocks, [2] first group
   unsigned long long* union_top;  // the commit's scratch top, zeroed by TouchRound
+  unsigned long long* round_ctr;  // [0] rounds that committed blocks, [1] groups scored
   Counters* ctr;
   int* err;
 };
@@ -174,5 +175,6 @@ __global__ void CommitKernel(CommitParams p);
 __global__ void TouchRound(RoundParams p);
 __global__ void FindCommitBlocks(RoundParams p);
 __global__ void MarkRound(RoundParams p);
+__global__ void SumGroupStats(const long long* slot_stats, const int32_t* glen, int ngroups, Counters* c);
 
 }  // namespace beam

@@ -37,8 +37,12 @@ constexpr int kTableSlots = 256;  // per-wave join table slots (long rows only)
 constexpr int kCandPhaseBytes = kSmallE * 4 + kSmallE * 2 * 2 + kSmallTC * 4 + (kSmallTC + 1) * 4 * 2;
 constexpr int kCandArrayBytes = kSmallC * (2 + 2 + 8 + 8) + (kSmallC + 1) * 4;
 constexpr int kTableBytes = kWaves * kTableSlots * 8;
-constexpr int kRegionBytes = ((kCandPhaseBytes > kCandArrayBytes + kTableBytes ? kCandPhaseBytes
-                                                                                : kCandArrayBytes + kTableBytes) + 15) & ~15;
+constexpr int kTableOffset = (kCandArrayBytes + 15) & ~15;
+constexpr int kRegionBytes = ((kCandPhaseBytes > kTableOffset + kTableBytes ? kCandPhaseBytes
+                                                                             : kTableOffset + kTableBytes) + 15) & ~15;
+// Rows with at most this many candidates rank them by counting against the
+// row's keys staged in the wave's join table (two 64-bit words per key).
+constexpr int kRankMax = kTableSlots * 8 / 16;
 static_assert(2 * kSmallC * 2 <= kTableBytes, "survivor lists reuse the join tables");
 
 struct GroupArrays {
@@ -110,182 +114,80 @@ __device__ __forceinline__ void TriDecode(uint32_t l, uint32_t s, uint32_t* x, u
   *y = static_cast<uint32_t>(static_cast<int64_t>(l) - row_start(xi) + xi + 1);
 }
 
-// Exact MAGS-compatible gain of members (a, b) from the staged rows, one wave
-// (quotient_graph.cpp:340-462).  The merge-join sums separate into per-row
-// streams plus a correction over common neighbours:
-//   before = sum_{x in A} BC(sa,sx,ea) + sum_{x in B} BC(sb,sx,eb) + internal terms
-//   after  = sum_{x in A} BC(sw,sx,ea) + sum_{x in B} BC(sw,sx,eb)
-//          + sum_{x in A and B} [BC(sw,sx,ea+eb) - BC(sw,sx,ea) - BC(sw,sx,eb)] + internal
-// Each staged entry carries its target's size, so no global memory is read.
-// Common neighbours are found by scanning the shorter row (broadcast reads)
-// or, for long rows, through a per-wave hash of the shorter row in chunks.
-__device__ void ScoreStaged(const GroupArrays& A, const int32_t* m_rep, const int32_t* m_size,
-                            const int32_t* m_len, const int32_t* m_int, const uint32_t* m_eoff, int a, int b,
-                            int32_t* tkey, int32_t* tcnt, int* err, long long* out_gain, long long* out_before) {
-  const int lane = LaneId();
-  const int la = m_len[a], lb = m_len[b];
-  const int s_idx = la <= lb ? a : b, l_idx = la <= lb ? b : a;
-  const int ls = m_len[s_idx], ll = m_len[l_idx];
-  const uint32_t so = m_eoff[s_idx], lo = m_eoff[l_idx];
-  const int s_rep = m_rep[s_idx], l_rep = m_rep[l_idx];
-  const int64_t ss = m_size[s_idx], sl = m_size[l_idx], sw = ss + sl;
-  int64_t before = 0, after = 0, eab_s = 0, eab_l = 0;
-  for (int k = lane; k < ls; k += kWave) {
-    const int t = A.ent_t[so + k];
-    const int64_t e = A.ent_e[so + k];
-    if (t == l_rep) {
-      eab_s = e;
-    } else {
-      const int64_t st = A.ent_st[so + k];
-      before += BlockCost(ss, st, e, false, err);
-      after += BlockCost(sw, st, e, false, err);
-    }
-  }
-  const bool direct = ls <= 48;
-  for (int k = lane; k < ll; k += kWave) {
-    const int t = A.ent_t[lo + k];
-    const int64_t e = A.ent_e[lo + k];
-    if (t == s_rep) {
-      eab_l = e;
-      continue;
-    }
-    const int64_t st = A.ent_st[lo + k];
-    before += BlockCost(sl, st, e, false, err);
-    after += BlockCost(sw, st, e, false, err);
-    if (direct) {
-      for (int j = 0; j < ls; ++j) {
-        if (A.ent_t[so + j] == t) {
-          const int64_t es = A.ent_e[so + j];
-          after += BlockCost(sw, st, es + e, false, err) - BlockCost(sw, st, es, false, err) -
-                   BlockCost(sw, st, e, false, err);
-          break;
-        }
-      }
-    }
-  }
-  if (!direct) {
-    const int chunk = kTableSlots / 2;
-    const uint32_t mask = kTableSlots - 1;
-    for (int c0 = 0; c0 < ls; c0 += chunk) {
-      const int cn = min(chunk, ls - c0);
-      for (int k = lane; k < kTableSlots; k += kWave) tkey[k] = -1;
-      WaveSync();
-      for (int k = lane; k < cn; k += kWave) {
-        const int t = A.ent_t[so + c0 + k];
-        if (t == l_rep) continue;
-        uint32_t h = HashInt(static_cast<uint32_t>(t)) & mask;
-        while (atomicCAS(&tkey[h], -1, t) != -1) h = (h + 1) & mask;
-        tcnt[h] = A.ent_e[so + c0 + k];
-      }
-      WaveSync();
-      for (int k = lane; k < ll; k += kWave) {
-        const int t = A.ent_t[lo + k];
-        if (t == s_rep) continue;
-        uint32_t h = HashInt(static_cast<uint32_t>(t)) & mask;
-        while (true) {
-          const int key = tkey[h];
-          if (key == t) {
-            const int64_t es = tcnt[h], e = A.ent_e[lo + k], st = A.ent_st[lo + k];
-            after += BlockCost(sw, st, es + e, false, err) - BlockCost(sw, st, es, false, err) -
-                     BlockCost(sw, st, e, false, err);
-            break;
-          }
-          if (key == -1) break;
-          h = (h + 1) & mask;
-        }
-      }
-      WaveSync();
-    }
-  }
-  before = WaveSum64(before);
-  after = WaveSum64(after);
-  eab_s = WaveSum64(eab_s);
-  eab_l = WaveSum64(eab_l);
-  if (lane == 0) {
-    if (eab_s != eab_l) atomicOr(err, kErrAsymmetric);
-    const int64_t sa = m_size[a], sb = m_size[b], ia = m_int[a], ib = m_int[b];
-    before += BlockCost(sa, sa, ia, true, err) + BlockCost(sb, sb, ib, true, err) +
-              BlockCost(sa, sb, eab_s, false, err);
-    after += BlockCost(sw, sw, ia + ib + eab_s, true, err);
-    *out_gain = before - after;
-    *out_before = before;
-  }
+// Proxy matrix words: the legacy proxy sum in the low 32 bits and the
+// common-neighbour correction of the pair's exact gain (signed) in the high
+// 32 bits, so one atomic add updates both.  Both fit: the proxy is at most
+// 1.5 x arcs and |correction| at most arcs, and arcs < 1.72e9 (row offsets
+// are 32-bit with 2.5 x arcs of room).
+__device__ __forceinline__ long long ProxyOf(long long w) {
+  return static_cast<long long>(static_cast<unsigned long long>(w) & 0xffffffffULL);
 }
+__device__ __forceinline__ long long CorrOf(long long w) { return w >> 32; }
 
-constexpr int kSeg = 16;  // lanes per pair in the small-pair scorer
-
-__device__ __forceinline__ int64_t SegSum64(int64_t v) {
+template <int kLanes>
+__device__ __forceinline__ int64_t LaneSum64(int64_t v) {
 #pragma unroll
-  for (int off = kSeg / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  for (int off = kLanes / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
   return v;
 }
 
-// True when the pair fits the 16-lane scorer: both rows at most 64 entries,
-// so each lane streams at most four entries of each row.
-__device__ __forceinline__ bool SmallPair(const int32_t* m_len, int a, int b) {
-  return m_len[a] <= 64 && m_len[b] <= 64;
-}
-
-// ScoreStaged on a 16-lane segment (four pairs per wave); common neighbours
-// are found by scanning the shorter row with broadcast reads.
-__device__ void ScoreSegment(const GroupArrays& A, const int32_t* m_rep, const int32_t* m_size,
-                             const int32_t* m_len, const int32_t* m_int, const uint32_t* m_eoff, int a, int b,
-                             bool active, int* err, long long* out_gain, long long* out_before) {
-  const int sl_lane = LaneId() & (kSeg - 1);
-  int64_t before = 0, after = 0, eab_s = 0, eab_l = 0;
-  int64_t sw = 0;
+// Exact MAGS-compatible gain of members (a, b) from the staged rows on
+// kLanes lanes (quotient_graph.cpp:340-462).  The merge-join sums separate
+// into per-row streams plus a correction over common neighbours:
+//   before = sum_{x in A} BC(sa,sx,ea) + sum_{x in B} BC(sb,sx,eb) + internal terms
+//   after  = sum_{x in A} BC(sw,sx,ea) + sum_{x in B} BC(sw,sx,eb)
+//          + sum_{x in A and B} [BC(sw,sx,ea+eb) - BC(sw,sx,ea) - BC(sw,sx,eb)] + internal
+// (x ranges over neighbours other than a and b).  The common-neighbour sum is
+// accumulated for every pair of members while the candidates are built (each
+// common neighbour is a target bucket both rows fall into), so a pair costs
+// one pass over its two staged rows.  Each staged entry carries its target's
+// size, so no global memory is read.
+template <int kLanes>
+__device__ void ScorePairLanes(const GroupArrays& A, const int32_t* m_rep, const int32_t* m_size,
+                               const int32_t* m_len, const int32_t* m_int, const uint32_t* m_eoff, int a, int b,
+                               long long corr, bool active, int* err, long long* out_gain, long long* out_before) {
+  const int sl_lane = LaneId() & (kLanes - 1);
+  int64_t before = 0, after = 0, eab_a = 0, eab_b = 0;
   if (active) {
     const int la = m_len[a], lb = m_len[b];
-    const int s_idx = la <= lb ? a : b, l_idx = la <= lb ? b : a;
-    const int ls = m_len[s_idx], ll = m_len[l_idx];
-    const uint32_t so = m_eoff[s_idx], lo = m_eoff[l_idx];
-    const int s_rep = m_rep[s_idx], l_rep = m_rep[l_idx];
-    const int64_t ss = m_size[s_idx], sl = m_size[l_idx];
-    sw = ss + sl;
-    for (int k = sl_lane; k < ls; k += kSeg) {
-      const int t = A.ent_t[so + k];
-      const int64_t e = A.ent_e[so + k];
-      if (t == l_rep) {
-        eab_s = e;
-      } else {
-        const int64_t st = A.ent_st[so + k];
-        before += BlockCost(ss, st, e, false, err);
-        after += BlockCost(sw, st, e, false, err);
-      }
-    }
-    for (int k = sl_lane; k < ll; k += kSeg) {
-      const int t = A.ent_t[lo + k];
-      const int64_t e = A.ent_e[lo + k];
-      if (t == s_rep) {
-        eab_l = e;
+    const uint32_t oa = m_eoff[a], ob = m_eoff[b];
+    const int rep_a = m_rep[a], rep_b = m_rep[b];
+    const int64_t sa = m_size[a], sb = m_size[b], sw = sa + sb;
+    for (int k = sl_lane; k < la + lb; k += kLanes) {
+      const bool in_a = k < la;
+      const uint32_t idx = in_a ? oa + k : ob + (k - la);
+      const int t = A.ent_t[idx];
+      const int64_t e = A.ent_e[idx];
+      if (t == (in_a ? rep_b : rep_a)) {
+        if (in_a) eab_a = e; else eab_b = e;
         continue;
       }
-      const int64_t st = A.ent_st[lo + k];
-      before += BlockCost(sl, st, e, false, err);
+      const int64_t st = A.ent_st[idx];
+      before += BlockCost(in_a ? sa : sb, st, e, false, err);
       after += BlockCost(sw, st, e, false, err);
-      for (int j = 0; j < ls; ++j) {
-        if (A.ent_t[so + j] == t) {
-          const int64_t es = A.ent_e[so + j];
-          after += BlockCost(sw, st, es + e, false, err) - BlockCost(sw, st, es, false, err) -
-                   BlockCost(sw, st, e, false, err);
-          break;
-        }
-      }
     }
   }
-  before = SegSum64(before);
-  after = SegSum64(after);
-  eab_s = SegSum64(eab_s);
-  eab_l = SegSum64(eab_l);
+  before = LaneSum64<kLanes>(before);
+  after = LaneSum64<kLanes>(after);
+  eab_a = LaneSum64<kLanes>(eab_a);
+  eab_b = LaneSum64<kLanes>(eab_b);
   if (active && sl_lane == 0) {
-    if (eab_s != eab_l) atomicOr(err, kErrAsymmetric);
-    const int64_t sa = m_size[a], sb = m_size[b], ia = m_int[a], ib = m_int[b];
+    if (eab_a != eab_b) atomicOr(err, kErrAsymmetric);
+    const int64_t sa = m_size[a], sb = m_size[b], ia = m_int[a], ib = m_int[b], sw = sa + sb;
     before += BlockCost(sa, sa, ia, true, err) + BlockCost(sb, sb, ib, true, err) +
-              BlockCost(sa, sb, eab_s, false, err);
-    after += BlockCost(sw, sw, ia + ib + eab_s, true, err);
+              BlockCost(sa, sb, eab_a, false, err);
+    after += corr + BlockCost(sw, sw, ia + ib + eab_a, true, err);
     *out_gain = before - after;
     *out_before = before;
   }
+}
+
+constexpr int kSeg = 16;  // lanes per pair in the short-pair scorer
+
+// True when the pair's two rows have at most 64 entries together, so a
+// 16-lane segment streams at most four entries per lane.
+__device__ __forceinline__ bool SmallPair(const int32_t* m_len, int a, int b) {
+  return m_len[a] + m_len[b] <= 64;
 }
 
 // Candidate generation, exact scoring and selection for group gidx; results
@@ -368,7 +270,7 @@ __device__ void ScoreGroup(const MergeParams& p, int gidx, int slot, int32_t* se
                             static_cast<uint32_t>(g) * static_cast<uint32_t>(g - 1) / 2);
   const bool small = E <= kSmallE && g <= kSmallG;
   GroupArrays A;
-  A.tables = reinterpret_cast<int32_t*>(s_region + kCandArrayBytes);
+  A.tables = reinterpret_cast<int32_t*>(s_region + kTableOffset);
   if (small) {
     A.ent_t = s_ent_t;
     A.ent_e = s_ent_e;
@@ -523,9 +425,16 @@ __device__ void ScoreGroup(const MergeParams& p, int gidx, int slot, int32_t* se
     const int64_t old_cost = EncodeCost(ra, sa, t, st, ea) + EncodeCost(rb, sb, t, st, eb);
     const int64_t new_cost = EncodeCost(ra, sa + sb, t, st, ea + eb);
     const int64_t delta = old_cost - new_cost;
-    if (delta > 0) {
+    // t is a common neighbour of members i and j: its term of their exact
+    // gain's common-neighbour correction goes into the high word.
+    const int64_t sw = sa + sb;
+    const int64_t corr = BlockCost(sw, st, ea + eb, false, p.err) - BlockCost(sw, st, ea, false, p.err) -
+                         BlockCost(sw, st, eb, false, p.err);
+    const unsigned long long add =
+        (delta > 0 ? static_cast<unsigned long long>(delta) : 0ULL) + (static_cast<unsigned long long>(corr) << 32);
+    if (add != 0) {
       const int lo = min(i, j), hi = max(i, j);
-      atomicAdd(reinterpret_cast<unsigned long long*>(&A.proxy[lo * g + hi]), static_cast<unsigned long long>(delta));
+      atomicAdd(reinterpret_cast<unsigned long long*>(&A.proxy[lo * g + hi]), add);
     }
   }
   // Phase 6: direct edges between members (sweg.cpp:1905-1924).
@@ -565,13 +474,13 @@ __device__ void ScoreGroup(const MergeParams& p, int gidx, int slot, int32_t* se
       int cnt = 0;
       for (int r = 0; r < per_lane; ++r) {
         const int j = lane + r * kWave;
-        if (j < g && j != i && A.proxy[min(i, j) * g + max(i, j)] > 0) ++cnt;
+        if (j < g && j != i && ProxyOf(A.proxy[min(i, j) * g + max(i, j)]) > 0) ++cnt;
       }
       cnt = WaveSum32(cnt);
       if (cnt <= topk) {
         for (int r = 0; r < per_lane; ++r) {
           const int j = lane + r * kWave;
-          if (j < g && j != i && A.proxy[min(i, j) * g + max(i, j)] > 0) {
+          if (j < g && j != i && ProxyOf(A.proxy[min(i, j) * g + max(i, j)]) > 0) {
             const uint32_t bit = static_cast<uint32_t>(min(i, j)) * g + static_cast<uint32_t>(max(i, j));
             atomicOr(&A.candbits[bit >> 5], 1U << (bit & 31));
           }
@@ -587,7 +496,7 @@ __device__ void ScoreGroup(const MergeParams& p, int gidx, int slot, int32_t* se
         pv[r] = 0;
         const int j = lane + r * kWave;
         if (r < per_lane && j < g && j != i) {
-          const long long v = A.proxy[min(i, j) * g + max(i, j)];
+          const long long v = ProxyOf(A.proxy[min(i, j) * g + max(i, j)]);
           if (v > 0) {
             long long denom = m_aoc[i] + m_aoc[j];
             if (denom < 1) denom = 1;
@@ -596,6 +505,51 @@ __device__ void ScoreGroup(const MergeParams& p, int gidx, int slot, int32_t* se
             live |= 1U << r;
           }
         }
+      }
+      if (cnt <= kRankMax) {
+        // Rank by counting: key = (ratio bits, proxy << 32 | ~rep_j), larger
+        // is better (positive doubles order as their bit patterns; the proxy
+        // is below 2^32, see ProxyOf).  The row's
+        // keys are compacted into the wave's table; a candidate is kept when
+        // fewer than k keys beat it.
+        unsigned long long* wk = reinterpret_cast<unsigned long long*>(A.tables + WaveId() * kTableSlots * 2);
+        unsigned long long kp[8];
+        int base = 0;
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+          kp[r] = (static_cast<unsigned long long>(pv[r]) << 32) |
+                  static_cast<unsigned long long>(0xffffffffU - static_cast<uint32_t>(m_rep[min(g - 1, lane + r * kWave)]));
+          if (r >= per_lane) continue;
+          const bool have = (live >> r) & 1U;
+          const unsigned long long mask = __ballot(have);
+          if (have) {
+            const int pos = base + __popcll(mask & ((1ULL << lane) - 1ULL));
+            wk[2 * pos] = static_cast<unsigned long long>(__double_as_longlong(ratio[r]));
+            wk[2 * pos + 1] = kp[r];
+          }
+          base += __popcll(mask);
+        }
+        WaveSync();
+        int better[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int e = 0; e < cnt; ++e) {
+          const unsigned long long er = wk[2 * e], ep = wk[2 * e + 1];
+#pragma unroll
+          for (int r = 0; r < 8; ++r) {
+            if (r >= per_lane) break;
+            const unsigned long long mr = static_cast<unsigned long long>(__double_as_longlong(ratio[r]));
+            better[r] += (er > mr || (er == mr && ep > kp[r])) ? 1 : 0;
+          }
+        }
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+          if (((live >> r) & 1U) && better[r] < topk) {
+            const int j = lane + r * kWave;
+            const uint32_t bit = static_cast<uint32_t>(min(i, j)) * g + static_cast<uint32_t>(max(i, j));
+            atomicOr(&A.candbits[bit >> 5], 1U << (bit & 31));
+          }
+        }
+        WaveSync();
+        continue;
       }
       for (int round = 0; round < topk; ++round) {
         double br = 0.0;
@@ -682,20 +636,20 @@ __device__ void ScoreGroup(const MergeParams& p, int gidx, int slot, int32_t* se
       const bool in_range = pi < C;
       const int a = in_range ? A.cand_a[pi] : 0, b = in_range ? A.cand_b[pi] : 0;
       const bool mine = in_range && SmallPair(m_len, a, b);
+      const long long corr = mine ? CorrOf(A.proxy[a * g + b]) : 0;
       long long gain = 0, before = 0;
-      ScoreSegment(A, m_rep, m_size, m_len, m_int, m_eoff, a, b, mine, p.err, &gain, &before);
+      ScorePairLanes<kSeg>(A, m_rep, m_size, m_len, m_int, m_eoff, a, b, corr, mine, p.err, &gain, &before);
       if (mine && (LaneId() & (kSeg - 1)) == 0) {
         A.cand_gain[pi] = gain;
         A.cand_before[pi] = before;
       }
     }
-    int32_t* tkey = A.tables + WaveId() * kTableSlots * 2;
-    int32_t* tcnt = tkey + kTableSlots;
     for (uint32_t pi = static_cast<uint32_t>(WaveId()); pi < C; pi += kWaves) {
       const int a = A.cand_a[pi], b = A.cand_b[pi];
       if (SmallPair(m_len, a, b)) continue;
+      const long long corr = CorrOf(A.proxy[a * g + b]);
       long long gain = 0, before = 0;
-      ScoreStaged(A, m_rep, m_size, m_len, m_int, m_eoff, a, b, tkey, tcnt, p.err, &gain, &before);
+      ScorePairLanes<kWave>(A, m_rep, m_size, m_len, m_int, m_eoff, a, b, corr, true, p.err, &gain, &before);
       if (LaneId() == 0) {
         A.cand_gain[pi] = gain;
         A.cand_before[pi] = before;
@@ -819,6 +773,7 @@ __global__ __launch_bounds__(kThreads) void MergeGroupsKernel(MergeParams p) {
   // results are kept per group.
   __shared__ int s_item;
   const int ntodo = p.round->ntodo;
+  if (ntodo == 0) return;
   for (;;) {
     if (threadIdx.x == 0) s_item = atomicAdd(&p.round->steal, 1);
     __syncthreads();

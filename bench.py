@@ -159,6 +159,9 @@ def main() -> int:
             "avg_launch_us": round(kp["group_kernel_ms"] * 1e3 / launches, 3),
             "launches": kp["group_kernel_launches"],
             "bytes_per_launch": round(bytes_total / launches, 1),
+            "rounds": kp["rounds"],
+            "rounds_committing": kp["rounds_committing"],
+            "groups_scored": kp["groups_scored"],
         }
 
     value = pairs_total / total_s

@@ -131,7 +131,9 @@ int beam_hip_profile(beam_hip_engine* engine, int32_t enable);
 
 /* Since the last reset: out[0] = summed group-kernel time in ms (profiling
  * on), out[1] = timed launches, out[2] = member rows read, out[3] = their
- * cross entries, out[4] = candidate pairs scored. */
+ * cross entries, out[4] = candidate pairs scored, out[5] = merge rounds
+ * launched, out[6] = rounds that committed blocks, out[7] = groups scored
+ * (first scorings and rescorings).  out needs 8 entries. */
 int beam_hip_kernel_profile(beam_hip_engine* engine, double* out);
 
 /* Phase trace of the group kernel for one iteration (0 disables): for every
