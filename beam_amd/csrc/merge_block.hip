@@ -43,7 +43,8 @@ constexpr int kRegionBytes = ((kCandPhaseBytes > kTableOffset + kTableBytes ? kC
 // Rows with at most this many candidates rank them by counting against the
 // row's keys staged in the wave's join table (two 64-bit words per key).
 constexpr int kRankMax = kTableSlots * 8 / 16;
-static_assert(2 * kSmallC * 2 <= kTableBytes, "survivor lists reuse the join tables");
+static_assert(kSmallC * 8 <= kTableBytes, "small groups' survivor keys reuse the join tables");
+static_assert(kMaxGroup * 16 * 8 <= kRegionBytes, "large groups' survivor keys fit the LDS region");
 
 struct GroupArrays {
   int32_t* ent_t;       // target rep per staged entry
@@ -60,8 +61,6 @@ struct GroupArrays {
   long long* cand_gain;
   long long* cand_before;
   uint32_t* pass;       // survivor flags, then their prefix (C + 1)
-  uint16_t* plist;      // survivors in candidate order
-  uint16_t* sorted;     // survivors in selection order
   int32_t* tables;      // per-wave join tables (always LDS)
   // Global-mode companions of the int16 LDS arrays.
   int32_t* g_ent_i;
@@ -193,7 +192,7 @@ __device__ __forceinline__ bool SmallPair(const int32_t* m_len, int a, int b) {
 // Candidate generation, exact scoring and selection for group gidx; results
 // go to sel_out (pairs), sel_count[slot], slot_stats[slot].  Every early exit
 // is taken by the whole workgroup.
-__device__ void ScoreGroup(const MergeParams& p, int gidx, int slot, int32_t* sel_out) {
+__device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int slot, int32_t* sel_out) {
   long long* trace = nullptr;
   const long long t_start = static_cast<long long>(wall_clock64());
   auto stamp = [&](int k) {
@@ -232,6 +231,9 @@ __device__ void ScoreGroup(const MergeParams& p, int gidx, int slot, int32_t* se
   __shared__ int16_t s_ent_i[kSmallE];
   __shared__ __align__(16) unsigned char s_region[kRegionBytes];
   __shared__ unsigned char s_used[kMaxGroup];
+  __shared__ unsigned long long s_chosen[kMaxSelect];
+  __shared__ unsigned long long s_gain, s_work, s_entries;
+  __shared__ int s_nchosen, s_live;
   __shared__ unsigned long long s_base;
   __shared__ int s_flag;
   __shared__ uint32_t s_counts[2];
@@ -289,14 +291,12 @@ __device__ void ScoreGroup(const MergeParams& p, int gidx, int slot, int32_t* se
     A.pass = reinterpret_cast<uint32_t*>(c); c += (kSmallC + 1) * 4;
     A.cand_a = reinterpret_cast<uint16_t*>(c); c += kSmallC * 2;
     A.cand_b = reinterpret_cast<uint16_t*>(c);
-    A.plist = reinterpret_cast<uint16_t*>(A.tables);
-    A.sorted = A.plist + kSmallC;
     A.g_ent_i = A.g_ent_rank = A.g_bucket = nullptr;
   } else {
     if (tid == 0) {
       const unsigned long long e64 = E, tc64 = TC, g64 = static_cast<unsigned long long>(g), c64 = cmax;
       const unsigned long long bytes = e64 * 4 * 7 + tc64 * 4 + (tc64 + 1) * 8 + 16 + g64 * g64 * 8 +
-                                       (g64 * g64 / 32 + 1) * 4 + 16 + c64 * (8 + 8 + 2 + 2 + 2 + 2) +
+                                       (g64 * g64 / 32 + 1) * 4 + 16 + c64 * (8 + 8 + 2 + 2) +
                                        (c64 + 1) * 4 + 64;
       const unsigned long long base = atomicAdd(p.scratch_top, (bytes + 255) & ~255ULL);
       s_base = base;
@@ -327,9 +327,7 @@ __device__ void ScoreGroup(const MergeParams& p, int gidx, int slot, int32_t* se
     A.cand_before = reinterpret_cast<long long*>(r); r += cmax * 8ULL;
     A.pass = reinterpret_cast<uint32_t*>(r); r += (cmax + 1) * 4ULL;
     A.cand_a = reinterpret_cast<uint16_t*>(r); r += cmax * 2ULL;
-    A.cand_b = reinterpret_cast<uint16_t*>(r); r += cmax * 2ULL;
-    A.plist = reinterpret_cast<uint16_t*>(r); r += cmax * 2ULL;
-    A.sorted = reinterpret_cast<uint16_t*>(r);
+    A.cand_b = reinterpret_cast<uint16_t*>(r);
   }
   int16_t* s_ent_rank = reinterpret_cast<int16_t*>(s_region + kSmallE * 4);
   int16_t* s_bucket = s_ent_rank + kSmallE;
@@ -551,6 +549,88 @@ __device__ void ScoreGroup(const MergeParams& p, int gidx, int slot, int32_t* se
         WaveSync();
         continue;
       }
+      {
+        // More candidates than the table holds: only keys at least the k-th
+        // best of the lanes' best keys (k distinct candidates) can be among
+        // the row's top k, so only those are ranked.  With more than 128
+        // candidates on at most 8 per lane, at least 17 lanes hold one.
+        unsigned long long* wk = reinterpret_cast<unsigned long long*>(A.tables + WaveId() * kTableSlots * 2);
+        unsigned long long kp[8], hr = 0, hp = 0;
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+          kp[r] = (static_cast<unsigned long long>(pv[r]) << 32) |
+                  static_cast<unsigned long long>(0xffffffffU - static_cast<uint32_t>(m_rep[min(g - 1, lane + r * kWave)]));
+          if (!((live >> r) & 1U)) continue;
+          const unsigned long long kr = static_cast<unsigned long long>(__double_as_longlong(ratio[r]));
+          if (kr > hr || (kr == hr && kp[r] > hp)) {
+            hr = kr;
+            hp = kp[r];
+          }
+        }
+        wk[2 * lane] = hr;
+        wk[2 * lane + 1] = hp;
+        WaveSync();
+        int hrank = 0;
+        for (int e = 0; e < kWave; ++e) {
+          const unsigned long long er = wk[2 * e], ep = wk[2 * e + 1];
+          hrank += (er > hr || (er == hr && ep > hp)) ? 1 : 0;
+        }
+        WaveSync();
+        const int cap = kRankMax - 1;  // the last key slot carries the threshold
+        if (lane == 0) {
+          wk[2 * cap] = 0;  // no threshold (fewer valid lanes than k): rank all, below
+          wk[2 * cap + 1] = 0;
+        }
+        WaveSync();
+        if (hr != 0 && hrank == topk - 1) {
+          wk[2 * cap] = hr;
+          wk[2 * cap + 1] = hp;
+        }
+        WaveSync();
+        const unsigned long long tr = wk[2 * cap], tp = wk[2 * cap + 1];
+        WaveSync();
+        uint32_t keep = 0;
+        int base = 0;
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+          if (r >= per_lane) continue;
+          const unsigned long long kr = static_cast<unsigned long long>(__double_as_longlong(ratio[r]));
+          const bool have = ((live >> r) & 1U) && (kr > tr || (kr == tr && kp[r] >= tp));
+          const unsigned long long mask = __ballot(have);
+          if (have) {
+            keep |= 1U << r;
+            const int pos = base + __popcll(mask & ((1ULL << lane) - 1ULL));
+            if (pos < cap) {
+              wk[2 * pos] = kr;
+              wk[2 * pos + 1] = kp[r];
+            }
+          }
+          base += __popcll(mask);
+        }
+        WaveSync();
+        if (base <= cap) {
+          int better[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+          for (int e = 0; e < base; ++e) {
+            const unsigned long long er = wk[2 * e], ep = wk[2 * e + 1];
+#pragma unroll
+            for (int r = 0; r < 8; ++r) {
+              if (r >= per_lane) break;
+              const unsigned long long mr = static_cast<unsigned long long>(__double_as_longlong(ratio[r]));
+              better[r] += (er > mr || (er == mr && ep > kp[r])) ? 1 : 0;
+            }
+          }
+#pragma unroll
+          for (int r = 0; r < 8; ++r) {
+            if (((keep >> r) & 1U) && better[r] < topk) {
+              const int j = lane + r * kWave;
+              const uint32_t bit = static_cast<uint32_t>(min(i, j)) * g + static_cast<uint32_t>(max(i, j));
+              atomicOr(&A.candbits[bit >> 5], 1U << (bit & 31));
+            }
+          }
+          WaveSync();
+          continue;
+        }
+      }
       for (int round = 0; round < topk; ++round) {
         double br = 0.0;
         long long bp = 0;
@@ -670,7 +750,12 @@ __device__ void ScoreGroup(const MergeParams& p, int gidx, int slot, int32_t* se
   }
 
   // Phase 10: threshold filter (sweg.cpp:2146-2186) with the reference's FP64
-  // ratio; survivors keep their candidate order.
+  // ratio.  Each survivor gets the selection key
+  //   (2^32 - 1 - gain) << 18 | a_idx << 9 | b_idx,
+  // ascending in (gain desc, a_idx asc, b_idx asc) (sweg.cpp:2217-2235); the
+  // gain is below 2^32 (it is at most `before`, at most 1.5 x arcs).
+  unsigned long long* skey = small ? reinterpret_cast<unsigned long long*>(A.tables)
+                                   : reinterpret_cast<unsigned long long*>(s_region);
   auto passes = [&](uint32_t pi) {
     const long long gn = A.cand_gain[pi], bf = A.cand_before[pi];
     return gn > 0 && bf > 0 && !(static_cast<double>(gn) / static_cast<double>(bf) < p.theta);
@@ -679,65 +764,135 @@ __device__ void ScoreGroup(const MergeParams& p, int gidx, int slot, int32_t* se
     if (A.cand_gain[pi] > 0) atomicAdd(&s_counts[0], 1U);
     A.pass[pi] = passes(pi) ? 1U : 0U;
   }
-  __syncthreads();
-  const uint32_t P = ScanInPlace(A.pass, static_cast<int>(C), wave_tot);
-  for (uint32_t pi = tid; pi < C; pi += kThreads) {
-    if (A.pass[pi + 1] != A.pass[pi]) A.plist[A.pass[pi]] = static_cast<uint16_t>(pi);
+  if (tid == 0) {
+    s_nchosen = 0;
+    s_gain = 0;
+    s_work = 0;
+    s_entries = 0;
   }
   __syncthreads();
-
-  // Phase 11: order survivors by (gain desc, a_idx asc, b_idx asc) and pick
-  // endpoint-disjoint pairs greedily (sweg.cpp:2217-2265).  The keep is the
-  // member at the lower group position.
-  for (uint32_t x = tid; x < P; x += kThreads) {
-    const int px = A.plist[x];
-    const long long gx = A.cand_gain[px];
-    const int ax = A.cand_a[px], bx = A.cand_b[px];
-    uint32_t rank = 0;
-    for (uint32_t y = 0; y < P; ++y) {
-      const int py = A.plist[y];
-      const long long gy = A.cand_gain[py];
-      const int ay = A.cand_a[py], by = A.cand_b[py];
-      if (gy > gx || (gy == gx && (ay < ax || (ay == ax && by < bx)))) ++rank;
+  const uint32_t P = ScanInPlace(A.pass, static_cast<int>(C), wave_tot);
+  if (P > static_cast<uint32_t>(small ? kTableBytes / 8 : kRegionBytes / 8)) {
+    // Only reachable with --top-k above 16 on groups of more than 64 members.
+    if (tid == 0) atomicOr(p.err, kErrSelection);
+    return;
+  }
+  for (uint32_t pi = tid; pi < C; pi += kThreads) {
+    if (A.pass[pi + 1] != A.pass[pi]) {
+      const unsigned long long gn = static_cast<unsigned long long>(A.cand_gain[pi]);
+      if (gn >> 32) atomicOr(p.err, kErrSelection);
+      skey[A.pass[pi]] = ((0xffffffffULL - gn) << 18) | (static_cast<unsigned long long>(A.cand_a[pi]) << 9) |
+                         static_cast<unsigned long long>(A.cand_b[pi]);
     }
-    A.sorted[rank] = static_cast<uint16_t>(px);
+  }
+  for (int i = tid; i < g; i += kThreads) atomicAdd(&s_entries, static_cast<unsigned long long>(m_len[i]));
+  __syncthreads();
+
+  // Phase 11: greedy endpoint-disjoint selection in key order
+  // (sweg.cpp:2237-2265), in rounds: a live pair whose key is the smallest
+  // among the live pairs at both of its endpoints is taken by the sequential
+  // greedy too (every better pair at its endpoints is already dead), and
+  // taking all such pairs at once leaves the greedy's choices unchanged.
+  // Pairs with a used endpoint die; survivors not taken are overlap rejects.
+  {
+    unsigned long long* best = reinterpret_cast<unsigned long long*>(m_aoc);  // free after top-k
+    constexpr unsigned long long kDead = ~0ULL;
+    if (P > 0 && P <= static_cast<uint32_t>(kWave)) {
+      // One pair per lane of wave 0, wave-synchronous.
+      if (WaveId() == 0) {
+        const int lane = LaneId();
+        unsigned long long k = lane < static_cast<int>(P) ? skey[lane] : kDead;
+        const int a = static_cast<int>((k >> 9) & 511), b = static_cast<int>(k & 511);
+        while (__any(k != kDead)) {
+          if (k != kDead) {
+            best[a] = kDead;
+            best[b] = kDead;
+          }
+          WaveSync();
+          if (k != kDead) {
+            atomicMin(&best[a], k);
+            atomicMin(&best[b], k);
+          }
+          WaveSync();
+          if (k != kDead && best[a] == k && best[b] == k) {
+            s_used[a] = 1;
+            s_used[b] = 1;
+            s_chosen[atomicAdd(&s_nchosen, 1)] = k;
+          }
+          WaveSync();
+          if (k != kDead && (s_used[a] || s_used[b])) k = kDead;
+        }
+      }
+    } else if (P > 0) {
+      for (;;) {
+        for (uint32_t x = tid; x < P; x += kThreads) {
+          const unsigned long long k = skey[x];
+          if (k == kDead) continue;
+          best[(k >> 9) & 511] = kDead;
+          best[k & 511] = kDead;
+        }
+        if (tid == 0) s_live = 0;
+        __syncthreads();
+        for (uint32_t x = tid; x < P; x += kThreads) {
+          const unsigned long long k = skey[x];
+          if (k == kDead) continue;
+          atomicMin(&best[(k >> 9) & 511], k);
+          atomicMin(&best[k & 511], k);
+        }
+        __syncthreads();
+        for (uint32_t x = tid; x < P; x += kThreads) {
+          const unsigned long long k = skey[x];
+          if (k == kDead) continue;
+          const int a = static_cast<int>((k >> 9) & 511), b = static_cast<int>(k & 511);
+          if (best[a] == k && best[b] == k) {
+            s_used[a] = 1;
+            s_used[b] = 1;
+            s_chosen[atomicAdd(&s_nchosen, 1)] = k;
+          }
+        }
+        __syncthreads();
+        for (uint32_t x = tid; x < P; x += kThreads) {
+          const unsigned long long k = skey[x];
+          if (k == kDead) continue;
+          if (s_used[(k >> 9) & 511] || s_used[k & 511]) {
+            skey[x] = kDead;
+          } else {
+            s_live = 1;
+          }
+        }
+        __syncthreads();
+        if (!s_live) break;
+      }
+    }
+    __syncthreads();
+  }
+  // Selected pairs in key order; the keep is the member at the lower group
+  // position.
+  const int chosen = s_nchosen;
+  for (int c = tid; c < chosen; c += kThreads) {
+    const unsigned long long k = s_chosen[c];
+    int rank = 0;
+    for (int d = 0; d < chosen; ++d) rank += s_chosen[d] < k ? 1 : 0;
+    const int a = static_cast<int>((k >> 9) & 511), b = static_cast<int>(k & 511);
+    sel_out[2 * rank] = m_rep[a];
+    sel_out[2 * rank + 1] = m_rep[b];
+    atomicAdd(&s_gain, 0xffffffffULL - (k >> 18));
+    atomicAdd(&s_work, static_cast<unsigned long long>(m_len[a] + m_len[b]));
   }
   __syncthreads();
   if (tid == 0) {
-    int chosen = 0;
-    unsigned long long overlap = 0, work = 0;
-    long long total = 0;
-    for (uint32_t x = 0; x < P; ++x) {
-      const int px = A.sorted[x];
-      const int a = A.cand_a[px], b = A.cand_b[px];
-      if (s_used[a] || s_used[b]) {
-        ++overlap;
-        continue;
-      }
-      s_used[a] = s_used[b] = 1;
-      const int keep = m_rep[a], remove = m_rep[b];
-      sel_out[2 * chosen] = keep;
-      sel_out[2 * chosen + 1] = remove;
-      work += static_cast<unsigned long long>(m_len[a] + m_len[b]);
-      total += A.cand_gain[px];
-      ++chosen;
-    }
     p.sel_count[slot] = chosen;
-    // Per-slot statistics; the window resolve adds them to the iteration
-    // counters only for blocks that are committed.
-    unsigned long long entries = 0;
-    for (int i = 0; i < g; ++i) entries += static_cast<unsigned long long>(m_len[i]);
     long long* st = p.slot_stats + static_cast<long long>(slot) * kSlotStats;
     st[kStatGroup] = g;
     st[kStatCand] = C;
     st[kStatProxy] = PT;
     st[kStatPositive] = s_counts[0];
     st[kStatPass] = P;
-    st[kStatOverlap] = static_cast<long long>(overlap);
+    st[kStatOverlap] = static_cast<long long>(P) - chosen;
     st[kStatChosen] = chosen;
-    st[kStatGain] = total;
-    st[kStatWork] = static_cast<long long>(work);
-    st[kStatEntries] = static_cast<long long>(entries);
+    st[kStatGain] = static_cast<long long>(s_gain);
+    st[kStatWork] = static_cast<long long>(s_work);
+    st[kStatEntries] = static_cast<long long>(s_entries);
   }
   __syncthreads();
 
@@ -755,33 +910,38 @@ __device__ void ScoreGroup(const MergeParams& p, int gidx, int slot, int32_t* se
 }  // namespace
 
 __global__ __launch_bounds__(kThreads) void MergeGroupsKernel(MergeParams p) {
-  if (p.round == nullptr) {
-    // Window mode: workgroup s scores group first_group + s into slot s.
+  // Window mode (round == nullptr): workgroup s scores group first_group + s
+  // into slot s.  Round mode: workgroups take queued groups until the queue
+  // is drained; results are kept per group.
+  const bool round = p.round != nullptr;
+  if (!round && threadIdx.x == 0) {
     const int slot = static_cast<int>(blockIdx.x);
-    if (threadIdx.x == 0) {
-      p.sel_count[slot] = 0;
-      p.slot_stats[static_cast<long long>(slot) * kSlotStats + kStatGroup] = 0;
-      if (slot == 0 && p.reset_top != nullptr) *p.reset_top = 0;
-      if (slot == 0 && p.reset_queue != nullptr) *p.reset_queue = 0;
-    }
-    const int gidx = p.first_group + slot;
-    if (gidx >= p.ngroups) return;
-    ScoreGroup(p, gidx, slot, p.sel + 2LL * slot * kMaxSelect);
-    return;
+    p.sel_count[slot] = 0;
+    p.slot_stats[static_cast<long long>(slot) * kSlotStats + kStatGroup] = 0;
+    if (slot == 0 && p.reset_top != nullptr) *p.reset_top = 0;
+    if (slot == 0 && p.reset_queue != nullptr) *p.reset_queue = 0;
   }
-  // Round mode: workgroups take queued groups until the queue is drained;
-  // results are kept per group.
   __shared__ int s_item;
-  const int ntodo = p.round->ntodo;
-  if (ntodo == 0) return;
-  for (;;) {
-    if (threadIdx.x == 0) s_item = atomicAdd(&p.round->steal, 1);
-    __syncthreads();
-    const int item = s_item;
-    __syncthreads();
-    if (item >= ntodo) break;
-    const int gidx = p.todo[item];
-    ScoreGroup(p, gidx, gidx, p.sel + p.gstart[gidx]);
+  const int ntodo = round ? p.round->ntodo : 1;
+  for (int pass = 0; pass < ntodo; ++pass) {
+    int gidx, slot;
+    int32_t* sel_out;
+    if (round) {
+      if (threadIdx.x == 0) s_item = atomicAdd(&p.round->steal, 1);
+      __syncthreads();
+      const int item = s_item;
+      __syncthreads();
+      if (item >= ntodo) break;
+      gidx = p.todo[item];
+      slot = gidx;
+      sel_out = p.sel + p.gstart[gidx];
+    } else {
+      slot = static_cast<int>(blockIdx.x);
+      gidx = p.first_group + slot;
+      if (gidx >= p.ngroups) break;
+      sel_out = p.sel + 2LL * slot * kMaxSelect;
+    }
+    ScoreGroup(p, gidx, slot, sel_out);
   }
 }
 
