@@ -29,6 +29,8 @@ WORKLOADS = {
     "fa": ("fa: Chung-Lu n=4,039 m=88,234 gamma 2.5, 10 planted communities, seed 1", 1),
     "am": ("am: LFR-style n=334,863 m=925,872, degree exp 2.8, max 549, mu 0.1, seed 2", 2),
     "yo": ("yo: Chung-Lu n=1,134,890 m=2,987,624 gamma 2.1, max expected degree 28,750, seed 3", 3),
+    "lj": ("lj: Chung-Lu n=3,997,962 m=34,681,189 gamma 2.3, 40% of edges in 50k overlapping communities "
+           "(sizes power law 1.6 in [5,500]), seed 4", 4),
 }
 REF_FLAGS = [
     "--merge-mode", "batch-ea-blocked", "--scoring-backend", "cpu", "--cost-objective", "mags-compatible",
