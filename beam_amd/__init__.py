@@ -114,6 +114,11 @@ def library() -> ctypes.CDLL:
         "beam_hip_partition_hash": (ctypes.c_uint64, [P, i32]),
         "beam_hip_trace": (ctypes.c_int, [P, i32]),
         "beam_hip_trace_fetch": (i64, [P, P, i64]),
+        "beam_hip_ranker_last_error": (ctypes.c_char_p, []),
+        "beam_hip_ranker_forward": (ctypes.c_int, [P, P, P, i32, P]),
+        "beam_hip_ranker_gradient": (ctypes.c_int, [P, P, P, P, i32, P, P]),
+        "beam_hip_ranker_adam": (ctypes.c_int, [P, P, P, P, i32, ctypes.c_float]),
+        "beam_hip_ranker_train": (ctypes.c_int, [P, P, P, P, i32, i32, i32, ctypes.c_float, ctypes.c_uint64, P]),
         "beam_graph_load": (P, [ctypes.c_char_p]),
         "beam_graph_last_error": (ctypes.c_char_p, []),
         "beam_graph_info": (None, [P, P, P, P]),
@@ -137,6 +142,8 @@ def exported_symbols() -> list[str]:
         "beam_hip_nnz", "beam_hip_download_state", "beam_hip_score_pairs", "beam_hip_merge_groups",
         "beam_hip_divide", "beam_hip_commit", "beam_hip_reset", "beam_hip_profile",
         "beam_hip_kernel_profile", "beam_hip_partition_hash", "beam_hip_trace", "beam_hip_trace_fetch",
+        "beam_hip_ranker_last_error", "beam_hip_ranker_forward", "beam_hip_ranker_gradient",
+        "beam_hip_ranker_adam", "beam_hip_ranker_train",
         "beam_graph_load", "beam_graph_last_error",
         "beam_graph_info", "beam_graph_csr", "beam_graph_free",
     ]
@@ -340,3 +347,56 @@ def partition_hash(labels: np.ndarray) -> int:
     """FNV-1a 64 over node labels (Sweg::PartitionHash, src/sweg.cpp:2920-2928)."""
     labels = np.ascontiguousarray(labels, np.int32)
     return int(library().beam_hip_partition_hash(_ptr(labels), len(labels)))
+
+
+# ---------------------------------------------------------------- ranker kernels
+
+RANKER_PARAMS = 19841
+RANKER_NORM = 50
+
+
+def _ranker_check(rc: int) -> None:
+    if rc < 0:
+        raise BeamHipError(library().beam_hip_ranker_last_error().decode())
+
+
+def ranker_forward(params: np.ndarray, norm: np.ndarray, feats: np.ndarray) -> np.ndarray:
+    """MLP predictions of feature rows (n x 24) on the GPU."""
+    params = np.ascontiguousarray(params, dtype=np.float32)
+    norm = np.ascontiguousarray(norm, dtype=np.float32)
+    feats = np.ascontiguousarray(feats, dtype=np.float32)
+    out = np.zeros(len(feats), dtype=np.float32)
+    _ranker_check(library().beam_hip_ranker_forward(_ptr(params), _ptr(norm), _ptr(feats), len(feats), _ptr(out)))
+    return out
+
+
+def ranker_gradient(params, norm, feats, target):
+    params = np.ascontiguousarray(params, dtype=np.float32)
+    norm = np.ascontiguousarray(norm, dtype=np.float32)
+    feats = np.ascontiguousarray(feats, dtype=np.float32)
+    target = np.ascontiguousarray(target, dtype=np.float32)
+    grad = np.zeros(RANKER_PARAMS, dtype=np.float32)
+    loss = np.zeros(1, dtype=np.float32)
+    _ranker_check(library().beam_hip_ranker_gradient(_ptr(params), _ptr(norm), _ptr(feats), _ptr(target), len(feats),
+                                                     _ptr(grad), _ptr(loss)))
+    return grad, float(loss[0])
+
+
+def ranker_adam(params, grad, m, v, step: int, lr: float):
+    params = np.array(params, dtype=np.float32)
+    m = np.array(m, dtype=np.float32)
+    v = np.array(v, dtype=np.float32)
+    grad = np.ascontiguousarray(grad, dtype=np.float32)
+    _ranker_check(library().beam_hip_ranker_adam(_ptr(params), _ptr(grad), _ptr(m), _ptr(v), step, lr))
+    return params, m, v
+
+
+def ranker_train(params, norm, feats, target, epochs: int, batch: int, lr: float, seed: int):
+    params = np.array(params, dtype=np.float32)
+    norm = np.array(norm, dtype=np.float32)
+    feats = np.ascontiguousarray(feats, dtype=np.float32)
+    target = np.ascontiguousarray(target, dtype=np.float32)
+    loss = np.zeros(1, dtype=np.float32)
+    _ranker_check(library().beam_hip_ranker_train(_ptr(params), _ptr(norm), _ptr(feats), _ptr(target), len(feats),
+                                                  epochs, batch, lr, seed, _ptr(loss)))
+    return params, norm, float(loss[0])

@@ -202,6 +202,33 @@ int beam_hip_divide(beam_hip_engine* engine, int32_t iter, int32_t* order, int32
  * src/sweg.cpp:2977-3015).  count <= 262144. */
 int beam_hip_commit(beam_hip_engine* engine, const int32_t* keeps, const int32_t* removes, int32_t count);
 
+/* ---- Learned pair ranker ----
+ * No counterpart in the reference: the ranker prunes each group's candidate
+ * pairs before exact scoring (every committed merge is still costed exactly).
+ * The network is 24 -> 128 -> 128 -> 1 (ReLU); params holds
+ * BEAM_HIP_RANKER_PARAMS floats laid out W1[24][128], b1[128], W2[128][128],
+ * b2[128], w3[128], b3; norm holds BEAM_HIP_RANKER_NORM floats: feature means,
+ * feature inverse standard deviations, target mean, target inverse standard
+ * deviation.  Targets are slog(gain) = sign(gain) * log1p(|gain|).  These
+ * standalone entry points run on device 0 and exist for checking the kernels
+ * (forward on BF16 MFMA, FP32 backward, Adam) against a CPU restatement. */
+#define BEAM_HIP_RANKER_PARAMS 19841
+#define BEAM_HIP_RANKER_NORM 50
+const char* beam_hip_ranker_last_error(void);
+/* Predictions (standardised target units) of n feature rows (n x 24). */
+int beam_hip_ranker_forward(const float* params, const float* norm, const float* feats, int32_t n, float* out);
+/* Gradient of the mean squared error of the standardised targets over the
+ * n rows (grad: BEAM_HIP_RANKER_PARAMS floats) and that error. */
+int beam_hip_ranker_gradient(const float* params, const float* norm, const float* feats, const float* target,
+                             int32_t n, float* grad, float* loss);
+/* One Adam step (step counts from 1; beta1 0.9, beta2 0.999, eps 1e-8) on
+ * params / m / v in place. */
+int beam_hip_ranker_adam(float* params, const float* grad, float* m, float* v, int32_t step, float lr);
+/* Sets norm from the rows' moments, then trains `epochs` shuffled passes of
+ * full batches; params and norm are updated in place, *loss = last batch's. */
+int beam_hip_ranker_train(float* params, float* norm, const float* feats, const float* target, int32_t n,
+                          int32_t epochs, int32_t batch, float lr, uint64_t seed, float* loss);
+
 /* ---- Graph input (host side) ---- */
 
 /* Loads an undirected single-edge list with the reference's semantics and

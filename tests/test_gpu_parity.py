@@ -232,7 +232,7 @@ GOLDEN_KEYS = [
 ]
 
 
-@pytest.mark.parametrize("name,seed", [("fa", 1), ("am", 2)])
+@pytest.mark.parametrize("name,seed", [("fa", 1), ("am", 2), ("yo", 3)])
 def test_end_to_end_matches_reference(graph_path, name, seed):
     print(f"case: end-to-end {name}", flush=True)
     gold = _golden(name)

@@ -1,0 +1,129 @@
+"""Learned pair ranker: the CPU restatement (oracle/mlp.py) pinned by finite
+differences and Adam's closed-form first step, then the HIP kernels (BF16
+MFMA forward, FP32 backward, Adam, training) against it."""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import beam_amd
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+import mlp  # noqa: E402  (test infrastructure)
+
+
+def _batch(n: int, seed: int):
+    rng = np.random.default_rng(seed)
+    feats = rng.normal(0.0, 1.0, (n, mlp.FEAT)).astype(np.float32) * 2.0 + 0.5
+    norm = np.zeros(mlp.NORM, dtype=np.float32)
+    norm[:mlp.FEAT] = rng.normal(0.0, 0.3, mlp.FEAT)
+    norm[mlp.FEAT:2 * mlp.FEAT] = rng.uniform(0.4, 0.9, mlp.FEAT)
+    norm[2 * mlp.FEAT] = 0.2
+    norm[2 * mlp.FEAT + 1] = 0.7
+    target = (np.tanh(feats[:, 0]) + 0.3 * feats[:, 5] * feats[:, 7] - 0.2 * feats[:, 11] ** 2).astype(np.float32)
+    return feats, norm, target
+
+
+def test_layout_constants_match_header():
+    with open(os.path.join(ROOT, "include", "beam_hip.h")) as f:
+        text = f.read()
+    assert f"#define BEAM_HIP_RANKER_PARAMS {mlp.PARAMS}" in text
+    assert f"#define BEAM_HIP_RANKER_NORM {mlp.NORM}" in text
+    assert beam_amd.RANKER_PARAMS == mlp.PARAMS and beam_amd.RANKER_NORM == mlp.NORM
+
+
+def test_oracle_gradient_matches_finite_differences():
+    feats, norm, target = _batch(24, 3)
+    params = mlp.init_params(5).astype(np.float64)
+    grad, _ = mlp.gradient(params, norm, feats, target, mixed=False)
+    rng = np.random.default_rng(11)
+    picks = np.concatenate([
+        rng.choice(np.arange(mlp.OFF_W1, mlp.OFF_B1), 40, replace=False),
+        rng.choice(np.arange(mlp.OFF_B1, mlp.OFF_W2), 10, replace=False),
+        rng.choice(np.arange(mlp.OFF_W2, mlp.OFF_B2), 40, replace=False),
+        rng.choice(np.arange(mlp.OFF_B2, mlp.OFF_W3), 10, replace=False),
+        rng.choice(np.arange(mlp.OFF_W3, mlp.OFF_B3), 10, replace=False),
+        [mlp.OFF_B3],
+    ])
+    h = 1e-6
+    bad = 0
+    for k in picks:
+        up, dn = params.copy(), params.copy()
+        up[k] += h
+        dn[k] -= h
+        fd = (mlp.loss(up, norm, feats, target, mixed=False) - mlp.loss(dn, norm, feats, target, mixed=False)) / (2 * h)
+        if abs(fd - grad[k]) > 1e-6 + 1e-4 * abs(fd):
+            bad += 1
+    assert bad <= 1  # a ReLU kink inside the step can spoil one difference
+
+
+def test_oracle_adam_first_step_is_sign_step():
+    rng = np.random.default_rng(2)
+    p = rng.normal(size=mlp.PARAMS).astype(np.float32)
+    g = rng.normal(size=mlp.PARAMS).astype(np.float32)
+    z = np.zeros(mlp.PARAMS, dtype=np.float32)
+    q, m, v = mlp.adam(p, g, z, z, 1, 1e-3)
+    # First step with zero moments: m_hat = g, v_hat = g^2, so the update is
+    # lr * g / (|g| + eps) ~ lr * sign(g).
+    np.testing.assert_allclose(q, p - 1e-3 * g / (np.abs(g) + 1e-8), rtol=0, atol=2e-7)
+    np.testing.assert_allclose(m, 0.1 * g, rtol=1e-6)
+    np.testing.assert_allclose(v, 0.001 * g * g, rtol=1e-5)
+
+
+@pytest.mark.gpu
+def test_forward_matches_oracle():
+    feats, norm, _ = _batch(1000, 7)
+    params = mlp.init_params(9)
+    want, _, _, _ = mlp.forward(params, norm, feats, mixed=True)
+    got = beam_amd.ranker_forward(params, norm, feats)
+    # FP32 accumulation order on MFMA vs FP64 here; BF16 operands identical.
+    np.testing.assert_allclose(got, want, rtol=2e-3, atol=2e-3)
+    # The BF16 rounding is real: an all-FP64 forward differs visibly.
+    exact, _, _, _ = mlp.forward(params.astype(np.float64), norm, feats, mixed=False)
+    assert np.max(np.abs(exact - want)) > 1e-3
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n", [1, 77, 4096])
+def test_gradient_matches_oracle(n):
+    feats, norm, target = _batch(n, 13 + n)
+    params = mlp.init_params(17)
+    want, want_loss = mlp.gradient(params, norm, feats, target, mixed=True)
+    got, got_loss = beam_amd.ranker_gradient(params, norm, feats, target)
+    assert got_loss == pytest.approx(want_loss, rel=1e-3, abs=1e-5)
+    for lo, hi in ((mlp.OFF_W1, mlp.OFF_B1), (mlp.OFF_B1, mlp.OFF_W2), (mlp.OFF_W2, mlp.OFF_B2),
+                   (mlp.OFF_B2, mlp.OFF_W3), (mlp.OFF_W3, mlp.OFF_B3), (mlp.OFF_B3, mlp.PARAMS)):
+        err = np.linalg.norm(got[lo:hi] - want[lo:hi])
+        assert err <= 2e-3 * np.linalg.norm(want[lo:hi]) + 1e-6, (lo, hi, err)
+
+
+@pytest.mark.gpu
+def test_adam_matches_oracle():
+    rng = np.random.default_rng(4)
+    p = rng.normal(size=mlp.PARAMS).astype(np.float32)
+    g = rng.normal(size=mlp.PARAMS).astype(np.float32)
+    m = rng.normal(size=mlp.PARAMS).astype(np.float32) * 0.1
+    v = np.abs(rng.normal(size=mlp.PARAMS)).astype(np.float32) * 0.01
+    want = mlp.adam(p, g, m, v, 7, 3e-3)
+    got = beam_amd.ranker_adam(p, g, m, v, 7, 3e-3)
+    for a, b in zip(got, want):
+        np.testing.assert_allclose(a, b, rtol=1e-5, atol=1e-7)
+
+
+@pytest.mark.gpu
+def test_training_fits_a_smooth_target():
+    feats, _, target = _batch(65536, 21)
+    params = mlp.init_params(23)
+    norm = np.zeros(mlp.NORM, dtype=np.float32)
+    before = None
+    trained, tnorm, last = beam_amd.ranker_train(params, norm, feats, target, epochs=8, batch=4096, lr=2e-3, seed=1)
+    # The normalisation is the samples' moments.
+    np.testing.assert_allclose(tnorm[:mlp.FEAT], feats.mean(axis=0), rtol=1e-3, atol=1e-3)
+    np.testing.assert_allclose(tnorm[mlp.FEAT:2 * mlp.FEAT], 1.0 / feats.std(axis=0), rtol=1e-3)
+    before = mlp.loss(params, tnorm, feats, target)
+    after = mlp.loss(trained, tnorm, feats, target)
+    assert after < 0.2 * before and after < 0.2, (before, after, last)
