@@ -465,6 +465,7 @@ __global__ __launch_bounds__(kThreads) void SumGroupStats(const long long* slot_
     acc[kStatGain] += static_cast<unsigned long long>(st[kStatGain]);
     acc[kStatWork] += static_cast<unsigned long long>(st[kStatWork]);
     acc[kStatEntries] += static_cast<unsigned long long>(st[kStatEntries]);
+    acc[kStatScored] += static_cast<unsigned long long>(st[kStatScored]);
     acc[kSlotStats - 1] += static_cast<unsigned long long>(g) * static_cast<unsigned long long>(g - 1) / 2;
     acc[kSlotStats - 2] += static_cast<unsigned long long>(g);
     mx = max(mx, static_cast<unsigned long long>(g));
@@ -480,10 +481,10 @@ __global__ __launch_bounds__(kThreads) void SumGroupStats(const long long* slot_
     atomicAdd(&c->raw_pair_count, s_acc[kSlotStats - 1]);
     atomicAdd(&c->candidate_pairs_after_prune, s_acc[kStatCand]);
     atomicAdd(&c->proxy_pairs_examined, s_acc[kStatProxy]);
-    atomicAdd(&c->exact_gain_calls, s_acc[kStatCand]);
+    atomicAdd(&c->exact_gain_calls, s_acc[kStatScored]);
     atomicAdd(&c->positive_gain_count, s_acc[kStatPositive]);
     atomicAdd(&c->above_threshold, s_acc[kStatPass]);
-    atomicAdd(&c->rejected_by_threshold, s_acc[kStatCand] - s_acc[kStatPass]);
+    atomicAdd(&c->rejected_by_threshold, s_acc[kStatScored] - s_acc[kStatPass]);
     atomicAdd(&c->rejected_by_overlap, s_acc[kStatOverlap]);
     atomicAdd(&c->selected, s_acc[kStatChosen]);
     atomicAdd(reinterpret_cast<unsigned long long*>(&c->total_local_gain), s_acc[kStatGain]);

@@ -27,6 +27,7 @@
 #pragma once
 
 #include "common.hpp"
+#include "ranker.hpp"
 
 namespace beam {
 
@@ -46,7 +47,8 @@ enum : int {
   kStatGain,        // sum of selected gains
   kStatWork,        // row entries of selected pairs (validation work)
   kStatEntries,     // cross entries of the member rows
-  kSlotStats = 12
+  kStatScored,      // candidate pairs scored exactly (after ranker pruning)
+  kSlotStats = 14   // the two last slots are the reduction's derived sums
 };
 
 struct QuotientView {
@@ -102,6 +104,26 @@ struct MergeParams {
   // and write sel / sel_count / slot_stats per group instead of per slot.
   RoundState* round;
   const int32_t* todo;
+  // Learned ranker (kRankOff: plain exact scoring of every candidate).
+  int ranker_mode;
+  RankerView ranker;
+  float keep_frac;              // fraction of each group's candidates kept
+  int min_keep;                 // groups with at most this many candidates keep all
+  int iter;
+  float* samples;               // collect: [cap][kFeat] raw features
+  float* sample_target;         // collect: [cap] slog(exact gain)
+  unsigned long long* sample_key;  // collect: [cap] the pair's sampling hash (orders the samples)
+  unsigned long long* sample_count;
+  unsigned long long sample_cap;
+  uint32_t sample_threshold;    // collect: a pair is sampled when its hash is below
+  long long* audit_out;         // audit: [item][2] best exact gain, 1 if that pair was kept
+};
+
+enum : int {
+  kRankOff = 0,
+  kRankCollect = 1,  // score every candidate; sample (features, gain) for training
+  kRankPrune = 2,    // score only each group's top predicted candidates
+  kRankAudit = 3,    // score every candidate, report whether the best was kept; no selection
 };
 
 // One merge round's bookkeeping kernels (see RoundState).

@@ -61,6 +61,7 @@ struct GroupArrays {
   long long* cand_gain;
   long long* cand_before;
   uint32_t* pass;       // survivor flags, then their prefix (C + 1)
+  uint32_t* sig;        // ranker: 64 one-byte min-hash bins per member (16 words)
   int32_t* tables;      // per-wave join tables (always LDS)
   // Global-mode companions of the int16 LDS arrays.
   int32_t* g_ent_i;
@@ -189,10 +190,84 @@ __device__ __forceinline__ bool SmallPair(const int32_t* m_len, int a, int b) {
   return m_len[a] + m_len[b] <= 64;
 }
 
+// What the ranker's features read from a staged group.
+struct FeatureSource {
+  const int32_t* size;
+  const int32_t* len;
+  const int32_t* internal;
+  const long long* aoc;
+  const long long* proxy;  // packed proxy words (g x g)
+  const uint32_t* sig;     // 16 words of min-hash bins per member
+  int g;
+  double theta;
+};
+
+__device__ __forceinline__ void GroupPairFeatures(const FeatureSource& s, int a, int b, float* f) {
+  PairFeatureInput in;
+  in.sa = s.size[a];
+  in.sb = s.size[b];
+  in.la = s.len[a];
+  in.lb = s.len[b];
+  in.ia = s.internal[a];
+  in.ib = s.internal[b];
+  in.aoc_a = s.aoc[a];
+  in.aoc_b = s.aoc[b];
+  const long long w = s.proxy[a * s.g + b];
+  in.proxy = ProxyOf(w);
+  in.corr = CorrOf(w);
+  int match = 0, both = 0;
+  for (int k = 0; k < 16; ++k) {
+    const uint32_t x = s.sig[a * 16 + k], y = s.sig[b * 16 + k];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const uint32_t xb = (x >> (8 * q)) & 0xffU, yb = (y >> (8 * q)) & 0xffU;
+      if (xb != 0xffU && yb != 0xffU) {
+        ++both;
+        match += xb == yb ? 1 : 0;
+      }
+    }
+  }
+  in.sig_match = match;
+  in.sig_both = both;
+  in.g = s.g;
+  in.theta = s.theta;
+  PairFeatures(in, f);
+}
+
+// Predicted gains of candidates 0 .. C-1 into pred[2 * pi], 32 per wave on
+// MFMA.
+__device__ __forceinline__ void RankerPredict(const FeatureSource& fsrc, const RankerView& rv, const uint16_t* cand_a,
+                                           const uint16_t* cand_b, uint32_t C, float* pred) {
+  for (uint32_t c0 = static_cast<uint32_t>(WaveId()) * 32; c0 < C; c0 += kWaves * 32) {
+    const int lane = LaneId(), half = lane >> 5;
+    const uint32_t pi = c0 + static_cast<uint32_t>(lane & 31);
+    float f[kFeat];
+    if (pi < C) {
+      GroupPairFeatures(fsrc, cand_a[pi], cand_b[pi], f);
+    } else {
+#pragma unroll
+      for (int k = 0; k < kFeat; ++k) f[k] = 0.0f;
+    }
+    float xk[12];
+#pragma unroll
+    for (int sgrp = 0; sgrp < 3; ++sgrp) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int f0 = 8 * sgrp + j, f1 = 8 * sgrp + 4 + j;
+        const float x0 = (f[f0] - rv.norm[f0]) * rv.norm[kFeat + f0];
+        const float x1 = (f[f1] - rv.norm[f1]) * rv.norm[kFeat + f1];
+        xk[sgrp * 4 + j] = pi < C ? (half ? x1 : x0) : 0.0f;
+      }
+    }
+    const float y = MlpForward32(rv, xk, nullptr, nullptr, 0, 0);
+    if (half == 0 && pi < C) pred[2 * pi] = y;
+  }
+}
+
 // Candidate generation, exact scoring and selection for group gidx; results
 // go to sel_out (pairs), sel_count[slot], slot_stats[slot].  Every early exit
 // is taken by the whole workgroup.
-__device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int slot, int32_t* sel_out) {
+__device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int slot, int32_t* sel_out, int item) {
   long long* trace = nullptr;
   const long long t_start = static_cast<long long>(wall_clock64());
   auto stamp = [&](int k) {
@@ -237,6 +312,9 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
   __shared__ unsigned long long s_base;
   __shared__ int s_flag;
   __shared__ uint32_t s_counts[2];
+  __shared__ uint32_t s_sig[kSmallG * 16];
+  __shared__ uint32_t s_prefix, s_need;
+  __shared__ unsigned long long s_best;
 
   // Phase 1: members, member lookup table, staged entry offsets.
   const int start = p.gstart[gidx];
@@ -292,12 +370,13 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
     A.cand_a = reinterpret_cast<uint16_t*>(c); c += kSmallC * 2;
     A.cand_b = reinterpret_cast<uint16_t*>(c);
     A.g_ent_i = A.g_ent_rank = A.g_bucket = nullptr;
+    A.sig = s_sig;
   } else {
     if (tid == 0) {
       const unsigned long long e64 = E, tc64 = TC, g64 = static_cast<unsigned long long>(g), c64 = cmax;
       const unsigned long long bytes = e64 * 4 * 7 + tc64 * 4 + (tc64 + 1) * 8 + 16 + g64 * g64 * 8 +
                                        (g64 * g64 / 32 + 1) * 4 + 16 + c64 * (8 + 8 + 2 + 2) +
-                                       (c64 + 1) * 4 + 64;
+                                       (c64 + 1) * 4 + g64 * 64 + 64;
       const unsigned long long base = atomicAdd(p.scratch_top, (bytes + 255) & ~255ULL);
       s_base = base;
       s_flag = base + bytes > p.scratch_cap ? 1 : 0;
@@ -327,7 +406,9 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
     A.cand_before = reinterpret_cast<long long*>(r); r += cmax * 8ULL;
     A.pass = reinterpret_cast<uint32_t*>(r); r += (cmax + 1) * 4ULL;
     A.cand_a = reinterpret_cast<uint16_t*>(r); r += cmax * 2ULL;
-    A.cand_b = reinterpret_cast<uint16_t*>(r);
+    A.cand_b = reinterpret_cast<uint16_t*>(r); r += cmax * 2ULL;
+    r = align8(r);
+    A.sig = reinterpret_cast<uint32_t*>(r);
   }
   int16_t* s_ent_rank = reinterpret_cast<int16_t*>(s_region + kSmallE * 4);
   int16_t* s_bucket = s_ent_rank + kSmallE;
@@ -692,7 +773,8 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
     m_rowcnt[lo] = cnt;
   }
   __syncthreads();
-  const uint32_t C = ScanInPlace(m_rowcnt, g, wave_tot);
+  uint32_t C = ScanInPlace(m_rowcnt, g, wave_tot);
+  const uint32_t C0 = C;
   for (int lo = tid; lo < g; lo += kThreads) {
     uint32_t pos = m_rowcnt[lo];
     for (int hi = lo + 1; hi < g; ++hi) {
@@ -705,6 +787,105 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
     }
   }
   __syncthreads();
+
+  // Learned ranker: features of a candidate pair from the staged group.
+  const FeatureSource fsrc{m_size, m_len, m_int, m_aoc, A.proxy, A.sig, g, p.theta};
+  if (p.ranker_mode != kRankOff) {
+    // One-permutation min-hash of every member's neighbours: 64 bins of
+    // 8-bit minima (0xff: empty), packed four to a word.
+    for (int k = tid; k < g * 16; k += kThreads) A.sig[k] = 0xffffffffU;
+    __syncthreads();
+    for (uint32_t e = tid; e < E; e += kThreads) {
+      if (A.ent_slot[e] < 0) continue;  // the doubled self entry
+      const int i = small ? s_ent_i[e] : A.g_ent_i[e];
+      int bin;
+      unsigned val;
+      SigSlot(A.ent_t[e], &bin, &val);
+      uint32_t* word = &A.sig[i * 16 + (bin >> 2)];
+      const int sh = (bin & 3) * 8;
+      uint32_t old = *word;
+      while (((old >> sh) & 0xffU) > val) {
+        const uint32_t prev = atomicCAS(word, old, (old & ~(0xffU << sh)) | (static_cast<uint32_t>(val) << sh));
+        if (prev == old) break;
+        old = prev;
+      }
+    }
+    __syncthreads();
+  }
+  if ((p.ranker_mode == kRankPrune || p.ranker_mode == kRankAudit) && C > 0) {
+    // Predicted gains of all candidates, 32 per wave on MFMA (held as floats
+    // in cand_before until the exact scoring overwrites it).
+    float* pred = reinterpret_cast<float*>(A.cand_before);
+    RankerPredict(fsrc, p.ranker, A.cand_a, A.cand_b, C, pred);
+    __syncthreads();
+    // Keep the top max(min(C, min_keep), ceil(C * keep_frac)) predictions:
+    // radix-select the threshold key, ties kept in candidate order.
+    const uint32_t keepn = min(C, max(min(C, static_cast<uint32_t>(p.min_keep)),
+                                      static_cast<uint32_t>(ceilf(static_cast<float>(C) * p.keep_frac))));
+    auto key_of = [&](uint32_t pi) {
+      const uint32_t u = __float_as_uint(pred[2 * pi]);
+      return (u & 0x80000000U) ? ~u : (u | 0x80000000U);
+    };
+    if (keepn >= C) {
+      for (uint32_t pi = tid; pi < C; pi += kThreads) A.pass[pi] = 1U;
+    } else {
+      uint32_t* hist = m_rowcnt;  // free once the candidates are listed
+      if (tid == 0) {
+        s_prefix = 0;
+        s_need = keepn;
+      }
+      for (int pass = 0; pass < 4; ++pass) {
+        const int shift = 24 - 8 * pass;
+        for (int k = tid; k < 256; k += kThreads) hist[k] = 0;
+        __syncthreads();
+        const uint32_t prefix = s_prefix;
+        const uint32_t hmask = pass == 0 ? 0U : (0xffffffffU << (shift + 8));
+        for (uint32_t pi = tid; pi < C; pi += kThreads) {
+          const uint32_t key = key_of(pi);
+          if ((key & hmask) == (prefix & hmask)) atomicAdd(&hist[(key >> shift) & 0xffU], 1U);
+        }
+        __syncthreads();
+        if (tid == 0) {
+          uint32_t need = s_need;
+          for (int d = 255; d >= 0; --d) {
+            if (hist[d] >= need) {
+              s_prefix = prefix | (static_cast<uint32_t>(d) << shift);
+              break;
+            }
+            need -= hist[d];
+          }
+          s_need = need;
+        }
+        __syncthreads();
+      }
+      const uint32_t T = s_prefix, need_eq = s_need;
+      for (uint32_t pi = tid; pi < C; pi += kThreads) A.pass[pi] = key_of(pi) == T ? 1U : 0U;
+      __syncthreads();
+      ScanInPlace(A.pass, static_cast<int>(C), wave_tot);
+      for (uint32_t pi = tid; pi < C; pi += kThreads) {
+        const uint32_t key = key_of(pi);
+        A.pass[pi] = (key > T || (key == T && A.pass[pi] < need_eq)) ? 1U : 0U;
+      }
+    }
+    __syncthreads();
+    if (p.ranker_mode == kRankPrune && keepn < C) {
+      // Compact the kept candidates (order preserved) through cand_gain.
+      for (uint32_t pi = tid; pi < C; pi += kThreads) {
+        A.cand_gain[pi] = (static_cast<long long>(A.cand_a[pi]) << 16) | A.cand_b[pi];
+      }
+      __syncthreads();
+      ScanInPlace(A.pass, static_cast<int>(C), wave_tot);
+      for (uint32_t pi = tid; pi < C; pi += kThreads) {
+        if (A.pass[pi + 1] != A.pass[pi]) {
+          const long long ab = A.cand_gain[pi];
+          A.cand_a[A.pass[pi]] = static_cast<uint16_t>(ab >> 16);
+          A.cand_b[A.pass[pi]] = static_cast<uint16_t>(ab & 0xffff);
+        }
+      }
+      __syncthreads();
+      C = keepn;
+    }
+  }
 
   stamp(10);
   // Phase 9: exact gain of every candidate from the staged rows: pairs of
@@ -738,6 +919,43 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
   }
   __syncthreads();
   stamp(11);
+  if (p.ranker_mode == kRankAudit) {
+    // The best exact gain of the group (lowest candidate index among ties)
+    // and whether the ranker kept that pair; no selection.
+    if (tid == 0) s_best = 0;
+    __syncthreads();
+    for (uint32_t pi = tid; pi < C; pi += kThreads) {
+      const long long gn = A.cand_gain[pi];
+      if (gn > 0) atomicMax(&s_best, (static_cast<unsigned long long>(gn) << 16) | (0xffffULL - pi));
+    }
+    __syncthreads();
+    if (tid == 0 && p.audit_out != nullptr) {
+      const unsigned long long b = s_best;
+      const uint32_t pi = static_cast<uint32_t>(0xffffULL - (b & 0xffffULL));
+      p.audit_out[2 * item] = static_cast<long long>(b >> 16);
+      p.audit_out[2 * item + 1] = b != 0 ? static_cast<long long>(A.pass[pi]) : 0;
+    }
+    __syncthreads();
+    return;
+  }
+  if (p.ranker_mode == kRankCollect && p.samples != nullptr) {
+    // Training samples: (features, slog exact gain) of a hashed subset of
+    // the scored pairs.
+    for (uint32_t pi = tid; pi < C; pi += kThreads) {
+      const int a = A.cand_a[pi], b = A.cand_b[pi];
+      const uint64_t h = SplitMix64(0x9e3779b97f4a7c15ULL * static_cast<uint64_t>(static_cast<uint32_t>(m_rep[a]) + 1) ^
+                                    0xbf58476d1ce4e5b9ULL * static_cast<uint64_t>(static_cast<uint32_t>(m_rep[b]) + 1) ^
+                                    0x94d049bb133111ebULL * static_cast<uint64_t>(p.iter + 1));
+      if (static_cast<uint32_t>(h >> 32) >= p.sample_threshold) continue;
+      const unsigned long long idx = atomicAdd(p.sample_count, 1ULL);
+      if (idx >= p.sample_cap) continue;
+      float f[kFeat];
+      GroupPairFeatures(fsrc, a, b, f);
+      for (int k = 0; k < kFeat; ++k) p.samples[idx * kFeat + k] = f[k];
+      p.sample_target[idx] = SLog(static_cast<double>(A.cand_gain[pi]));
+      p.sample_key[idx] = h;
+    }
+  }
   if (p.dbg_count != nullptr) {
     const long long base = static_cast<long long>(slot) * p.dbg_cap;
     for (uint32_t pi = tid; pi < C && static_cast<int>(pi) < p.dbg_cap; pi += kThreads) {
@@ -884,7 +1102,8 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
     p.sel_count[slot] = chosen;
     long long* st = p.slot_stats + static_cast<long long>(slot) * kSlotStats;
     st[kStatGroup] = g;
-    st[kStatCand] = C;
+    st[kStatCand] = C0;
+    st[kStatScored] = C;
     st[kStatProxy] = PT;
     st[kStatPositive] = s_counts[0];
     st[kStatPass] = P;
@@ -924,12 +1143,12 @@ __global__ __launch_bounds__(kThreads) void MergeGroupsKernel(MergeParams p) {
   __shared__ int s_item;
   const int ntodo = round ? p.round->ntodo : 1;
   for (int pass = 0; pass < ntodo; ++pass) {
-    int gidx, slot;
+    int gidx, slot, item;
     int32_t* sel_out;
     if (round) {
       if (threadIdx.x == 0) s_item = atomicAdd(&p.round->steal, 1);
       __syncthreads();
-      const int item = s_item;
+      item = s_item;
       __syncthreads();
       if (item >= ntodo) break;
       gidx = p.todo[item];
@@ -937,11 +1156,12 @@ __global__ __launch_bounds__(kThreads) void MergeGroupsKernel(MergeParams p) {
       sel_out = p.sel + p.gstart[gidx];
     } else {
       slot = static_cast<int>(blockIdx.x);
+      item = slot;
       gidx = p.first_group + slot;
       if (gidx >= p.ngroups) break;
       sel_out = p.sel + 2LL * slot * kMaxSelect;
     }
-    ScoreGroup(p, gidx, slot, sel_out);
+    ScoreGroup(p, gidx, slot, sel_out, item);
   }
 }
 

@@ -68,8 +68,9 @@ __device__ __forceinline__ float MlpForward32(const RankerView& w, const float* 
 #pragma unroll
     for (int j = 0; j < 4; ++j) xb[s][j] = static_cast<short>(F32ToBf16(xk[s * 4 + j]));
   }
-  // Layer 1: H1^T[128 x 32] = W1T[128 x 24] X^T[24 x 32].
-  f32x16_t a1[4];
+  // Layer 1: H1^T[128 x 32] = W1T[128 x 24] X^T[24 x 32]; kept as packed
+  // BF16 pairs (the next layer's operands) to hold register pressure down.
+  uint32_t h1p[4][8];
 #pragma unroll
   for (int m = 0; m < 4; ++m) {
     f32x16_t acc = {};
@@ -80,26 +81,34 @@ __device__ __forceinline__ float MlpForward32(const RankerView& w, const float* 
       acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(wa, xb[s], acc, 0, 0, 0);
     }
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
+    for (int i = 0; i < 16; i += 2) {
       const int unit = m * 32 + 8 * (i >> 2) + 4 * half + (i & 3);
-      const float v = fmaxf(acc[i] + w.params[kOffB1 + unit], 0.0f);
-      acc[i] = v;
-      if (h1 != nullptr && col < npairs) h1[static_cast<long long>(base + col) * kHid + unit] = v;
+      const float v0 = fmaxf(acc[i] + w.params[kOffB1 + unit], 0.0f);
+      const float v1 = fmaxf(acc[i + 1] + w.params[kOffB1 + unit + 1], 0.0f);
+      if (h1 != nullptr && col < npairs) {
+        h1[static_cast<long long>(base + col) * kHid + unit] = v0;
+        h1[static_cast<long long>(base + col) * kHid + unit + 1] = v1;
+      }
+      h1p[m][i >> 1] = static_cast<uint32_t>(F32ToBf16(v0)) | (static_cast<uint32_t>(F32ToBf16(v1)) << 16);
     }
-    a1[m] = acc;
   }
-  // Layer 2: H2^T[128 x 32] = W2T[128 x 128] H1^T[128 x 32]; layer 3 folded in.
+  // Layer 2: H2^T[128 x 32] = W2T[128 x 128] H1^T[128 x 32]; layer 3 folded
+  // in.  The output tiles run one after another (not unrolled) so that only
+  // one tile's operands are in flight.
   float y = 0.0f;
-#pragma unroll
+#pragma unroll 1
   for (int m = 0; m < 4; ++m) {
     f32x16_t acc = {};
 #pragma unroll
     for (int s = 0; s < 16; ++s) {
       const bf16x4_t wa =
           *reinterpret_cast<const bf16x4_t*>(w.w2t + (m * 32 + col) * kHid + s * 8 + half * 4);
+      const uint32_t lo = h1p[s >> 2][2 * (s & 3)], hi = h1p[s >> 2][2 * (s & 3) + 1];
       bf16x4_t hb;
-#pragma unroll
-      for (int j = 0; j < 4; ++j) hb[j] = static_cast<short>(F32ToBf16(a1[s >> 2][4 * (s & 3) + j]));
+      hb[0] = static_cast<short>(lo & 0xffffU);
+      hb[1] = static_cast<short>(lo >> 16);
+      hb[2] = static_cast<short>(hi & 0xffffU);
+      hb[3] = static_cast<short>(hi >> 16);
       acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(wa, hb, acc, 0, 0, 0);
     }
 #pragma unroll
@@ -127,35 +136,44 @@ struct PairFeatureInput {
   double theta;            // threshold of the iteration
 };
 
+__device__ __forceinline__ float SLogf(float x) { return x >= 0.0f ? log1pf(x) : -log1pf(-x); }
+
+// The 24 features (FP32): sizes, degrees, internal counts, approximate old
+// costs, the legacy proxy and its ratio, the exact gain's common-neighbour
+// correction, min-hash neighbour overlap, group size and threshold, and a few
+// ratios of these.
 __device__ __forceinline__ void PairFeatures(const PairFeatureInput& in, float* f) {
-  const double sa = static_cast<double>(in.sa), sb = static_cast<double>(in.sb);
-  const double la = static_cast<double>(in.la), lb = static_cast<double>(in.lb);
-  const double denom = static_cast<double>(max(static_cast<int64_t>(1), in.aoc_a + in.aoc_b));
-  const double jac = in.sig_both > 0 ? static_cast<double>(in.sig_match) / in.sig_both : 0.0;
-  f[0] = SLog(sa);
-  f[1] = SLog(sb);
-  f[2] = SLog(la);
-  f[3] = SLog(lb);
-  f[4] = SLog(static_cast<double>(in.ia));
-  f[5] = SLog(static_cast<double>(in.ib));
-  f[6] = SLog(static_cast<double>(in.aoc_a));
-  f[7] = SLog(static_cast<double>(in.aoc_b));
-  f[8] = SLog(static_cast<double>(in.proxy));
-  f[9] = static_cast<float>(static_cast<double>(in.proxy) / denom);
-  f[10] = SLog(static_cast<double>(in.corr));
-  f[11] = static_cast<float>(jac);
-  f[12] = SLog(jac * fmin(la, lb));
-  f[13] = SLog(static_cast<double>(in.g));
+  const float sa = static_cast<float>(in.sa), sb = static_cast<float>(in.sb);
+  const float la = static_cast<float>(in.la), lb = static_cast<float>(in.lb);
+  const float ia = static_cast<float>(in.ia), ib = static_cast<float>(in.ib);
+  const float aa = static_cast<float>(in.aoc_a), ab = static_cast<float>(in.aoc_b);
+  const float px = static_cast<float>(in.proxy), cr = static_cast<float>(in.corr);
+  const float denom = static_cast<float>(max(static_cast<int64_t>(1), in.aoc_a + in.aoc_b));
+  const float jac = in.sig_both > 0 ? static_cast<float>(in.sig_match) / static_cast<float>(in.sig_both) : 0.0f;
+  f[0] = SLogf(sa);
+  f[1] = SLogf(sb);
+  f[2] = SLogf(la);
+  f[3] = SLogf(lb);
+  f[4] = SLogf(ia);
+  f[5] = SLogf(ib);
+  f[6] = SLogf(aa);
+  f[7] = SLogf(ab);
+  f[8] = SLogf(px);
+  f[9] = px / denom;
+  f[10] = SLogf(cr);
+  f[11] = jac;
+  f[12] = SLogf(jac * fminf(la, lb));
+  f[13] = SLogf(static_cast<float>(in.g));
   f[14] = static_cast<float>(in.theta);
-  f[15] = SLog(sa * sb);
-  f[16] = static_cast<float>(la / fmax(1.0, la + lb));
-  f[17] = static_cast<float>(sa / (sa + sb));
-  f[18] = SLog(la + lb);
-  f[19] = SLog(static_cast<double>(in.proxy - in.corr));
-  f[20] = static_cast<float>(static_cast<double>(in.ia) / fmax(1.0, sa * (sa - 1.0) * 0.5));
-  f[21] = static_cast<float>(static_cast<double>(in.ib) / fmax(1.0, sb * (sb - 1.0) * 0.5));
-  f[22] = SLog(static_cast<double>(in.aoc_a + in.aoc_b));
-  f[23] = static_cast<float>(static_cast<double>(in.proxy) / fmax(1.0, la + lb));
+  f[15] = SLogf(sa * sb);
+  f[16] = la / fmaxf(1.0f, la + lb);
+  f[17] = sa / (sa + sb);
+  f[18] = SLogf(la + lb);
+  f[19] = SLogf(static_cast<float>(in.proxy - in.corr));
+  f[20] = ia / fmaxf(1.0f, sa * (sa - 1.0f) * 0.5f);
+  f[21] = ib / fmaxf(1.0f, sb * (sb - 1.0f) * 0.5f);
+  f[22] = SLogf(static_cast<float>(in.aoc_a + in.aoc_b));
+  f[23] = px / fmaxf(1.0f, la + lb);
 }
 
 // One-permutation min-hash of a neighbour id (64 bins, 8-bit values).
