@@ -68,6 +68,10 @@ class Options(ctypes.Structure):
         ("ea_use_threshold", ctypes.c_int32),
         ("device", ctypes.c_int32),
         ("seed", ctypes.c_uint64),
+        ("ranker", ctypes.c_int32),
+        ("ranker_train_iters", ctypes.c_int32),
+        ("ranker_keep", ctypes.c_float),
+        ("ranker_recall_min", ctypes.c_float),
     ]
 
 
@@ -119,6 +123,9 @@ def library() -> ctypes.CDLL:
         "beam_hip_ranker_gradient": (ctypes.c_int, [P, P, P, P, i32, P, P]),
         "beam_hip_ranker_adam": (ctypes.c_int, [P, P, P, P, i32, ctypes.c_float]),
         "beam_hip_ranker_train": (ctypes.c_int, [P, P, P, P, i32, i32, i32, ctypes.c_float, ctypes.c_uint64, P]),
+        "beam_hip_ranker_stats": (ctypes.c_int, [P, P]),
+        "beam_hip_ranker_get": (ctypes.c_int, [P, P, P]),
+        "beam_hip_ranker_set": (ctypes.c_int, [P, P, P]),
         "beam_graph_load": (P, [ctypes.c_char_p]),
         "beam_graph_last_error": (ctypes.c_char_p, []),
         "beam_graph_info": (None, [P, P, P, P]),
@@ -143,7 +150,8 @@ def exported_symbols() -> list[str]:
         "beam_hip_divide", "beam_hip_commit", "beam_hip_reset", "beam_hip_profile",
         "beam_hip_kernel_profile", "beam_hip_partition_hash", "beam_hip_trace", "beam_hip_trace_fetch",
         "beam_hip_ranker_last_error", "beam_hip_ranker_forward", "beam_hip_ranker_gradient",
-        "beam_hip_ranker_adam", "beam_hip_ranker_train",
+        "beam_hip_ranker_adam", "beam_hip_ranker_train", "beam_hip_ranker_stats", "beam_hip_ranker_get",
+        "beam_hip_ranker_set",
         "beam_graph_load", "beam_graph_last_error",
         "beam_graph_info", "beam_graph_csr", "beam_graph_free",
     ]
@@ -191,10 +199,13 @@ class Engine:
     """Device-resident merge engine over one graph (one GPU)."""
 
     def __init__(self, graph: Graph, seed: int, top_k: int = 16, group_batch_size: int = 64,
-                 divide_hash_dims: int = 16, divide_max_group: int = 512, device: int = 0):
+                 divide_hash_dims: int = 16, divide_max_group: int = 512, device: int = 0,
+                 ranker: bool = False, ranker_train_iters: int = 5, ranker_keep: float = 0.25,
+                 ranker_recall_min: float = 0.9):
         self.lib = library()
         self.n = graph.n
-        opt = Options(top_k, group_batch_size, divide_hash_dims, divide_max_group, 1, device, seed)
+        opt = Options(top_k, group_batch_size, divide_hash_dims, divide_max_group, 1, device, seed,
+                      1 if ranker else 0, ranker_train_iters, ranker_keep, ranker_recall_min)
         h = ctypes.c_void_p()
         row_ptr = np.ascontiguousarray(graph.row_ptr, dtype=np.int64)
         col = np.ascontiguousarray(graph.col, dtype=np.int32)
@@ -243,6 +254,24 @@ class Engine:
         out = np.empty(self.n, dtype=np.int32)
         _check(self.lib.beam_hip_labels(self.h, _ptr(out)))
         return out
+
+    def ranker_stats(self) -> dict:
+        out = np.zeros(11, dtype=np.float64)
+        _check(self.lib.beam_hip_ranker_stats(self.h, _ptr(out)))
+        names = ["enabled", "ready", "samples", "train_loss", "train_ms", "audited", "audit_hits", "fallback_iter",
+                 "pruned_iters", "candidate_pairs", "scored_pairs"]
+        return {k: (float(v) if k in ("train_loss", "train_ms") else int(v)) for k, v in zip(names, out)}
+
+    def ranker_get(self) -> tuple[np.ndarray, np.ndarray]:
+        params = np.zeros(RANKER_PARAMS, dtype=np.float32)
+        norm = np.zeros(RANKER_NORM, dtype=np.float32)
+        _check(self.lib.beam_hip_ranker_get(self.h, _ptr(params), _ptr(norm)))
+        return params, norm
+
+    def ranker_set(self, params: np.ndarray, norm: np.ndarray) -> None:
+        params = np.ascontiguousarray(params, dtype=np.float32)
+        norm = np.ascontiguousarray(norm, dtype=np.float32)
+        _check(self.lib.beam_hip_ranker_set(self.h, _ptr(params), _ptr(norm)))
 
     def supernodes(self) -> tuple[np.ndarray, np.ndarray]:
         """(offsets, members): supernodes by ascending representative, members

@@ -11,6 +11,7 @@
 #include "../../include/beam_hip.h"
 #include "engine.hpp"
 #include "prims.hpp"
+#include "ranker_host.hpp"
 #include "score_pair.hpp"
 
 namespace beam {
@@ -21,6 +22,14 @@ thread_local std::string g_error;
 
 constexpr int kMaxIterations = 4096;
 constexpr int kCommitGrid = 512;
+// Learned ranker constants.
+constexpr int kRankerMinKeep = 8;                 // groups with at most this many candidates keep all
+constexpr int kAuditGroups = 1024;                // groups audited per pruned iteration
+constexpr int kAuditMinApplicable = 32;           // fewer applicable audits: no decision
+constexpr unsigned long long kSampleCap = 1ULL << 20;
+constexpr int kTrainEpochs = 4;
+constexpr int kTrainBatch = 65536;
+constexpr float kTrainLr = 1e-3f;
 
 template <typename T>
 T* DevAlloc(size_t count) {
@@ -250,6 +259,23 @@ __global__ void InitRound(RoundState* rs, int end, int32_t* naff, unsigned long 
   *group_top = 0;
 }
 
+// Sort records of the training samples by their sampling hash.
+__global__ void SampleRecs(int n, const unsigned long long* key, SortRec* recs) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  recs[i] = SortRec{key[i], 0U, i};
+}
+
+// Samples in key order.
+__global__ void GatherSamples(int n, const SortRec* recs, const float* feats, const float* target, float* feats_out,
+                              float* target_out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const long long src = recs[i].val;
+  for (int k = 0; k < kFeat; ++k) feats_out[static_cast<long long>(i) * kFeat + k] = feats[src * kFeat + k];
+  target_out[i] = target[src];
+}
+
 // Queues the groups [0, last) with at least two members for scoring.
 __global__ void QueueGroups(int last, const int32_t* glen, int32_t* gmark, int stamp, int32_t* todo, RoundState* rs) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
@@ -431,6 +457,28 @@ struct Engine {
   unsigned long long* touch = nullptr;
   int32_t* commit_out = nullptr;  // [0] committed slots, [1] committed blocks, [2] first group
   int32_t commit_host[2] = {0, 0};
+
+  // Learned ranker (options.ranker).
+  RankerState ranker;
+  bool ranker_ready = false;       // trained (or imported) network
+  bool ranker_imported = false;    // network set from outside: no training
+  bool ranker_fallback = false;    // audit failed: plain from then on
+  int ranker_mode_now = kRankOff;  // mode of the current iteration's rounds
+  float* samples = nullptr;        // [sample_cap][kFeat]
+  float* sample_target = nullptr;  // [sample_cap]
+  unsigned long long* sample_key = nullptr;
+  unsigned long long* sample_count = nullptr;
+  unsigned long long sample_cap = 0;
+  uint32_t sample_threshold = 0;
+  long long* audit_out = nullptr;  // [kAuditGroups][3]
+  int32_t* audit_list = nullptr;
+  RoundState* audit_rs = nullptr;
+  int64_t last_pairs = 0;          // candidates scored in the previous iteration
+  // Ranker statistics since the last reset.
+  long long rk_samples = 0, rk_audited = 0, rk_hits = 0, rk_pruned_iters = 0, rk_fallback_iter = 0;
+  double rk_loss = 0, rk_train_ms = 0;
+  void RankerSetupIteration(int iter, int G, double theta);
+  void RankerTrain();
   int32_t* node_group = nullptr;  // rounds: group of every active rep
   int32_t* gmark = nullptr;       // rounds: epoch of the queue holding each group
   int32_t* todo[2] = {nullptr, nullptr};
@@ -486,6 +534,21 @@ struct Engine {
   int Divide(int iter);
   void MergeBlocks(int iter, int ngroups, double theta, Counters* c);
   void LaunchRound(int G, int nblocks, int block_size, double theta, Counters* c);
+  void SetRankerParams(MergeParams& mp) {
+    mp.ranker_mode = ranker_mode_now;
+    mp.ranker = ranker.View();
+    mp.keep_frac = opt.ranker_keep;
+    mp.min_keep = kRankerMinKeep;
+    mp.iter = cur_iter;
+    mp.samples = samples;
+    mp.sample_target = sample_target;
+    mp.sample_key = sample_key;
+    mp.sample_count = sample_count;
+    mp.sample_cap = sample_cap;
+    mp.sample_threshold = sample_threshold;
+    mp.audit_out = audit_out;
+  }
+  int cur_iter = 0;
   void LaunchWindow(int first_group, int nslots, int ngroups, double theta, Counters* c,
                     const int32_t* order_arr, const int32_t* gstart_arr, const int32_t* glen_arr, int32_t* dbg_pairs,
                     long long* dbg_gain, int32_t* dbg_count, int dbg_cap);
@@ -506,7 +569,8 @@ Engine::~Engine() {
                   open_rep, mh,       recs[0],  recs[1],   flag,     excl,       rid,       run_start, run_open,
                   run_seg,  elem_open, new_idx, gflag,     gstart,   glen,       d_total,   d_maxlen, sel,
                   sel_count, mark_dest, dest,   mark_keep, mark_row, aff,        naff,      scratch_a, scratch_u,
-                  slot_stats, touch,  commit_out, node_group, gmark,     todo[0],   todo[1], rs, round_ctr,
+                  slot_stats, touch,  commit_out, node_group, gmark,     todo[0],   todo[1], rs, round_ctr, samples, sample_target, sample_key,
+                  sample_count, audit_out, audit_list, audit_rs,
                   tops,     ctr,      err,      enc,        trace_buf};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -597,6 +661,19 @@ void Engine::Create(const int64_t* h_row_ptr, const int32_t* h_col, int n_, long
     HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, MergeGroupsKernel, 512, 0));
     score_grid = std::max(1, cus) * std::max(1, per_cu);
   }
+  if (opt.ranker) {
+    if (opt.ranker_train_iters <= 0) opt.ranker_train_iters = 5;
+    if (opt.ranker_keep <= 0.0f) opt.ranker_keep = 0.25f;
+    if (opt.ranker_keep > 1.0f) throw std::invalid_argument("ranker_keep must be in (0, 1]");
+    sample_cap = kSampleCap;
+    samples = DevAlloc<float>(sample_cap * kFeat);
+    sample_target = DevAlloc<float>(sample_cap);
+    sample_key = DevAlloc<unsigned long long>(sample_cap);
+    sample_count = DevAlloc<unsigned long long>(1);
+    audit_out = DevAlloc<long long>(3 * kAuditGroups);
+    audit_list = DevAlloc<int32_t>(kAuditGroups);
+    audit_rs = DevAlloc<RoundState>(1);
+  }
   mark_dest = DevAlloc<int32_t>(n);
   dest = DevAlloc<int32_t>(n);
   mark_keep = DevAlloc<int32_t>(n);
@@ -644,6 +721,18 @@ void Engine::Reset() {
   blocks_run = 0;
   rounds_run = 0;
   blocks_per_round = 32.0;
+  if (opt.ranker) {
+    if (!ranker_imported) {
+      ranker.Init(opt.seed, stream);
+      ranker_ready = false;
+    }
+    ranker_fallback = false;
+    ranker_mode_now = kRankOff;
+    HIP_CHECK(hipMemsetAsync(sample_count, 0, sizeof(unsigned long long), stream));
+    last_pairs = 0;
+    rk_samples = rk_audited = rk_hits = rk_pruned_iters = rk_fallback_iter = 0;
+    rk_loss = rk_train_ms = 0;
+  }
   divide_max_group = 0;
   divide_fallback_splits = 0;
   round_rows.clear();
@@ -853,6 +942,7 @@ void Engine::LaunchRound(int G, int nblocks, int block_size, double theta, Count
   mp.trace = (trace_buf != nullptr && tracing_now) ? trace_buf : nullptr;
   mp.round = cur;
   mp.todo = todo[epoch & 1];
+  SetRankerParams(mp);
   if (profile) {
     if (ev_used + 2 > ev_pool.size()) {
       for (int k = 0; k < 256; ++k) {
@@ -971,6 +1061,118 @@ void Engine::EnsurePoolRoom() {
   Sync();
 }
 
+// The ranker's mode for this iteration: sample during the first
+// ranker_train_iters iterations, train once after them, then audit a sample
+// of groups before pruning each later iteration (a failed audit falls back
+// to plain exact scoring for the rest of the run).
+void Engine::RankerSetupIteration(int iter, int G, double theta) {
+  ranker_mode_now = kRankOff;
+  if (!opt.ranker || ranker_fallback || G <= 0) return;
+  if (ranker_imported && iter <= opt.ranker_train_iters) return;  // installed network: plain warm-up
+  if (!ranker_ready && iter <= opt.ranker_train_iters) {
+    const double quota = static_cast<double>(sample_cap) / opt.ranker_train_iters;
+    const double expected = last_pairs > 0 ? static_cast<double>(last_pairs) : 16.0 * std::max(1, last_active);
+    const double prob = std::min(1.0, quota / std::max(1.0, expected));
+    sample_threshold = prob >= 1.0 ? 0xffffffffU : static_cast<uint32_t>(prob * 4294967296.0);
+    ranker_mode_now = kRankCollect;
+    return;
+  }
+  if (!ranker_ready) RankerTrain();
+  if (!ranker_ready) {
+    ranker_fallback = true;  // too few samples to train on
+    rk_fallback_iter = iter;
+    return;
+  }
+  // Audit: groups 0, s, 2s, ... scored both ways on the iteration-start state.
+  const int stride = std::max(1, G / kAuditGroups);
+  const int count = std::min(kAuditGroups, (G + stride - 1) / stride);
+  std::vector<int32_t> list(static_cast<size_t>(count));
+  for (int k = 0; k < count; ++k) list[static_cast<size_t>(k)] = k * stride;
+  HIP_CHECK(hipMemcpyAsync(audit_list, list.data(), sizeof(int32_t) * count, hipMemcpyHostToDevice, stream));
+  HIP_CHECK(hipMemsetAsync(audit_out, 0, sizeof(long long) * 3 * count, stream));
+  const RoundState init{0, 0, 0, count, 0, {0, 0, 0}};
+  HIP_CHECK(hipMemcpyAsync(audit_rs, &init, sizeof(init), hipMemcpyHostToDevice, stream));
+  HIP_CHECK(hipMemsetAsync(&tops[0], 0, sizeof(unsigned long long), stream));
+  MergeParams mp{};
+  mp.q = QuotientView{pool[pool_idx], row_off, row_len, size, internal};
+  mp.order = order;
+  mp.gstart = gstart;
+  mp.glen = glen;
+  mp.ngroups = G;
+  mp.top_k = opt.top_k;
+  mp.theta = theta;
+  mp.sel = sel;
+  mp.sel_count = sel_count;
+  mp.slot_stats = slot_stats;
+  mp.scratch = scratch_a;
+  mp.scratch_cap = scratch_a_cap;
+  mp.scratch_top = &tops[0];
+  mp.ctr = ctr;
+  mp.err = err;
+  mp.round = audit_rs;
+  mp.todo = audit_list;
+  ranker_mode_now = kRankAudit;
+  SetRankerParams(mp);
+  MergeGroupsKernel<<<score_grid, 512, 0, stream>>>(mp);
+  std::vector<long long> out(static_cast<size_t>(3 * count));
+  HIP_CHECK(hipMemcpyAsync(out.data(), audit_out, sizeof(long long) * 3 * count, hipMemcpyDeviceToHost, stream));
+  Sync();
+  CheckDeviceError("ranker audit");
+  long long applicable = 0, hits = 0;
+  for (int k = 0; k < count; ++k) {
+    if (out[static_cast<size_t>(3 * k)] > 0 && out[static_cast<size_t>(3 * k + 2)] != 0) {
+      ++applicable;
+      hits += out[static_cast<size_t>(3 * k + 1)] != 0 ? 1 : 0;
+    }
+  }
+  rk_audited += applicable;
+  rk_hits += hits;
+  if (applicable >= kAuditMinApplicable &&
+      static_cast<double>(hits) < static_cast<double>(opt.ranker_recall_min) * static_cast<double>(applicable)) {
+    ranker_fallback = true;
+    rk_fallback_iter = iter;
+    ranker_mode_now = kRankOff;
+    return;
+  }
+  ranker_mode_now = kRankPrune;
+  ++rk_pruned_iters;
+}
+
+// Trains the network on the collected samples, taken in the order of their
+// sampling hash so that the run is deterministic.
+void Engine::RankerTrain() {
+  unsigned long long count = 0;
+  HIP_CHECK(hipMemcpyAsync(&count, sample_count, sizeof(count), hipMemcpyDeviceToHost, stream));
+  Sync();
+  const int n = static_cast<int>(std::min(count, sample_cap));
+  rk_samples = n;
+  if (n < 1024) return;
+  const auto t0 = std::chrono::steady_clock::now();
+  SortRec* r0 = DevAlloc<SortRec>(static_cast<size_t>(n));
+  SortRec* r1 = DevAlloc<SortRec>(static_cast<size_t>(n));
+  float* f_sorted = DevAlloc<float>(static_cast<size_t>(n) * kFeat);
+  float* t_sorted = DevAlloc<float>(static_cast<size_t>(n));
+  try {
+    SampleRecs<<<Blocks(n, 256), 256, 0, stream>>>(n, sample_key, r0);
+    SortRec* sorted = SegmentedKeySort(r0, r1, n, sort, stream);
+    GatherSamples<<<Blocks(n, 256), 256, 0, stream>>>(n, sorted, samples, sample_target, f_sorted, t_sorted);
+    rk_loss = ranker.Train(f_sorted, t_sorted, n, kTrainEpochs, kTrainBatch, kTrainLr, opt.seed, stream);
+    Sync();
+  } catch (...) {
+    (void)hipFree(r0);
+    (void)hipFree(r1);
+    (void)hipFree(f_sorted);
+    (void)hipFree(t_sorted);
+    throw;
+  }
+  (void)hipFree(r0);
+  (void)hipFree(r1);
+  (void)hipFree(f_sorted);
+  (void)hipFree(t_sorted);
+  rk_train_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  ranker_ready = true;
+}
+
 void Engine::Iterate(int iter) {
   if (iter < 1 || iter > kMaxIterations) throw std::invalid_argument("iteration out of range");
   const auto t0 = std::chrono::steady_clock::now();
@@ -979,6 +1181,8 @@ void Engine::Iterate(int iter) {
   EnsurePoolRoom();
   const double theta = 1.0 / static_cast<double>(iter + 1);  // reciprocal policy (sweg.cpp:400-406)
   Counters* c = ctr + iter;
+  cur_iter = iter;
+  RankerSetupIteration(iter, G, theta);
   tracing_now = trace_iter == iter;
   if (tracing_now) {
     if (trace_buf) (void)hipFree(trace_buf);
@@ -1038,6 +1242,7 @@ void Engine::Iterate(int iter) {
   Counters h{};
   HIP_CHECK(hipMemcpy(&h, c, sizeof(h), hipMemcpyDeviceToHost));
   iter_ctr.push_back(h);
+  last_pairs = static_cast<int64_t>(h.candidate_pairs_after_prune);
   member_rows += h.member_rows;
   member_entries += h.member_entries;
   pairs_scored += h.exact_gain_calls;
@@ -1292,6 +1497,49 @@ int beam_hip_kernel_profile(beam_hip_engine* engine, double* out) {
     out[5] = static_cast<double>(e.rounds_run);
     out[6] = static_cast<double>(rc[0]);
     out[7] = static_cast<double>(rc[1]);
+    return 0;
+  });
+}
+
+int beam_hip_ranker_stats(beam_hip_engine* engine, double* out) {
+  return Guard([&] {
+    const auto& e = engine->e;
+    long long cand = 0, scored = 0;
+    for (const auto& c : e.iter_ctr) {
+      cand += static_cast<long long>(c.candidate_pairs_after_prune);
+      scored += static_cast<long long>(c.exact_gain_calls);
+    }
+    out[0] = e.opt.ranker ? 1.0 : 0.0;
+    out[1] = e.ranker_ready ? 1.0 : 0.0;
+    out[2] = static_cast<double>(e.rk_samples);
+    out[3] = e.rk_loss;
+    out[4] = e.rk_train_ms;
+    out[5] = static_cast<double>(e.rk_audited);
+    out[6] = static_cast<double>(e.rk_hits);
+    out[7] = static_cast<double>(e.rk_fallback_iter);
+    out[8] = static_cast<double>(e.rk_pruned_iters);
+    out[9] = static_cast<double>(cand);
+    out[10] = static_cast<double>(scored);
+    return 0;
+  });
+}
+
+int beam_hip_ranker_get(beam_hip_engine* engine, float* params, float* norm) {
+  return Guard([&] {
+    auto& e = engine->e;
+    if (!e.opt.ranker) throw std::invalid_argument("the engine was created without the ranker");
+    e.ranker.Download(params, norm, e.stream);
+    return 0;
+  });
+}
+
+int beam_hip_ranker_set(beam_hip_engine* engine, const float* params, const float* norm) {
+  return Guard([&] {
+    auto& e = engine->e;
+    if (!e.opt.ranker) throw std::invalid_argument("the engine was created without the ranker");
+    e.ranker.Upload(params, norm, e.stream);
+    e.ranker_imported = true;
+    e.ranker_ready = true;
     return 0;
   });
 }

@@ -116,7 +116,8 @@ struct MergeParams {
   unsigned long long* sample_count;
   unsigned long long sample_cap;
   uint32_t sample_threshold;    // collect: a pair is sampled when its hash is below
-  long long* audit_out;         // audit: [item][2] best exact gain, 1 if that pair was kept
+  long long* audit_out;         // audit: [item][3] best exact gain, 1 if that pair was kept,
+                                //        1 if the ranker dropped any candidate
 };
 
 enum : int {

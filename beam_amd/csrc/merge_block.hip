@@ -314,6 +314,7 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
   __shared__ uint32_t s_counts[2];
   __shared__ uint32_t s_sig[kSmallG * 16];
   __shared__ uint32_t s_prefix, s_need;
+  __shared__ int s_pruned;
   __shared__ unsigned long long s_best;
 
   // Phase 1: members, member lookup table, staged entry offsets.
@@ -822,6 +823,7 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
     // radix-select the threshold key, ties kept in candidate order.
     const uint32_t keepn = min(C, max(min(C, static_cast<uint32_t>(p.min_keep)),
                                       static_cast<uint32_t>(ceilf(static_cast<float>(C) * p.keep_frac))));
+    if (tid == 0) s_pruned = keepn < C ? 1 : 0;
     auto key_of = [&](uint32_t pi) {
       const uint32_t u = __float_as_uint(pred[2 * pi]);
       return (u & 0x80000000U) ? ~u : (u | 0x80000000U);
@@ -932,8 +934,9 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
     if (tid == 0 && p.audit_out != nullptr) {
       const unsigned long long b = s_best;
       const uint32_t pi = static_cast<uint32_t>(0xffffULL - (b & 0xffffULL));
-      p.audit_out[2 * item] = static_cast<long long>(b >> 16);
-      p.audit_out[2 * item + 1] = b != 0 ? static_cast<long long>(A.pass[pi]) : 0;
+      p.audit_out[3 * item] = static_cast<long long>(b >> 16);
+      p.audit_out[3 * item + 1] = b != 0 ? static_cast<long long>(A.pass[pi]) : 0;
+      p.audit_out[3 * item + 2] = C > 0 ? s_pruned : 0;
     }
     __syncthreads();
     return;

@@ -22,6 +22,8 @@ namespace beam {
 namespace {
 
 constexpr int kWavesPerBlock = 4;
+constexpr int kRedBlocks = 512;      // blocks of the per-column batch reductions
+constexpr int kMomentBlocks = 256;
 
 // Rows r of a batch take sample idx[r] (or r without idx).
 __global__ __launch_bounds__(kWavesPerBlock * kWave) void RankerForwardKernel(RankerView w, const float* feats,
@@ -68,9 +70,13 @@ __global__ void LossGradKernel(const float* pred, const float* target, const int
   if (threadIdx.x == 0) atomicAdd(loss, s_sum);
 }
 
-// dW3 += sum_r dy_r h2_r, db3 += sum_r dy_r, dH2 = dy w3 [h2 > 0].
+// Reductions over the batch write one partial row per block and are summed
+// in block order afterwards (ReduceRowsKernel), so training is deterministic.
+
+// Partial [blk][kHid + 1] of dW3 = sum_r dy_r h2_r and db3 = sum_r dy_r;
+// dH2 = dy w3 [h2 > 0].
 __global__ void OutputGradKernel(const float* dy, const float* h2, const float* params, int n, float* dh2,
-                                 float* grad) {
+                                 float* part) {
   const int tid = static_cast<int>(threadIdx.x);  // blockDim = kHid
   float acc = 0.0f, accb = 0.0f;
   const float w3 = params[kOffW3 + tid];
@@ -81,8 +87,8 @@ __global__ void OutputGradKernel(const float* dy, const float* h2, const float* 
     dh2[static_cast<long long>(r) * kHid + tid] = h > 0.0f ? d * w3 : 0.0f;
     if (tid == 0) accb += d;
   }
-  atomicAdd(&grad[kOffW3 + tid], acc);
-  if (tid == 0) atomicAdd(&grad[kOffB3], accb);
+  part[blockIdx.x * (kHid + 1) + tid] = acc;
+  if (tid == 0) part[blockIdx.x * (kHid + 1) + kHid] = accb;
 }
 
 // dH1[r][i] = [h1 > 0] sum_j dH2[r][j] W2[i][j], 32 rows per block.
@@ -110,8 +116,8 @@ __global__ __launch_bounds__(256) void HiddenGradKernel(const float* dh2, const 
   }
 }
 
-// G[i][j] += sum_r A[r][i] D[r][j] over one chunk of rows per block
-// (A: n x m, D: n x kHid, G: m x kHid), 32 x 32 output tile per block.
+// Partial [chunk][m x kHid] of G[i][j] = sum_r A[r][i] D[r][j] over one
+// chunk of rows per block (A: n x m, D: n x kHid), 32 x 32 tile per block.
 constexpr int kGemmChunk = 1024;
 __global__ __launch_bounds__(256) void GemmAtBKernel(const float* a, int m, const float* d, int n, float* g) {
   __shared__ float s_a[64][33];
@@ -138,17 +144,28 @@ __global__ __launch_bounds__(256) void GemmAtBKernel(const float* a, int m, cons
     __syncthreads();
   }
   if (i0 + oi < m) {
+    float* out = g + static_cast<long long>(blockIdx.x) * m * kHid;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) atomicAdd(&g[(i0 + oi) * kHid + j0 + oj + q], acc[q]);
+    for (int q = 0; q < 4; ++q) out[(i0 + oi) * kHid + j0 + oj + q] = acc[q];
   }
 }
 
-// g[j] += sum_r D[r][j] (D: n x kHid).
-__global__ void ColSumKernel(const float* d, int n, float* g) {
+// Partial [blk][kHid] of sum_r D[r][j] (D: n x kHid).
+__global__ void ColSumKernel(const float* d, int n, float* part) {
   const int j = static_cast<int>(threadIdx.x);  // blockDim = kHid
   float acc = 0.0f;
   for (int r = blockIdx.x; r < n; r += gridDim.x) acc += d[static_cast<long long>(r) * kHid + j];
-  atomicAdd(&g[j], acc);
+  part[blockIdx.x * kHid + j] = acc;
+}
+
+// out[c] = sum over rows r (in order) of part[r][c].
+template <typename T>
+__global__ void ReduceRowsKernel(const T* part, int rows, int width, T* out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= width) return;
+  T acc = 0;
+  for (int r = 0; r < rows; ++r) acc += part[static_cast<long long>(r) * width + c];
+  out[c] = acc;
 }
 
 __global__ void AdamKernel(float* params, const float* grad, float* m, float* v, int n, float lr, float b1, float b2,
@@ -175,8 +192,10 @@ __global__ void PackWeightsKernel(const float* params, unsigned short* w1t, unsi
   }
 }
 
-// acc[f] += x, acc[kFeat + f] += x^2 over features (the target's two last).
-__global__ __launch_bounds__(256) void MomentsKernel(const float* feats, const float* target, int n, double* acc) {
+// Partial [blk][2 kFeat + 2]: sums of x then x^2 per feature, the target's
+// two last.
+__global__ __launch_bounds__(256) void MomentsKernel(const float* feats, const float* target, int n, double* part) {
+  __shared__ double s[8][2 * kFeat + 2];
   const int tid = static_cast<int>(threadIdx.x);
   const int f = tid & 31, rl = tid >> 5;  // 8 rows in flight per block
   double s1 = 0.0, s2 = 0.0;
@@ -186,8 +205,14 @@ __global__ __launch_bounds__(256) void MomentsKernel(const float* feats, const f
       s1 += x;
       s2 += x * x;
     }
-    atomicAdd(&acc[f < kFeat ? f : 2 * kFeat], s1);
-    atomicAdd(&acc[f < kFeat ? kFeat + f : 2 * kFeat + 1], s2);
+    s[rl][f < kFeat ? f : 2 * kFeat] = s1;
+    s[rl][f < kFeat ? kFeat + f : 2 * kFeat + 1] = s2;
+  }
+  __syncthreads();
+  if (tid < 2 * kFeat + 2) {
+    double t = 0.0;
+    for (int r = 0; r < 8; ++r) t += s[r][tid];
+    part[blockIdx.x * (2 * kFeat + 2) + tid] = t;
   }
 }
 
@@ -217,7 +242,7 @@ T* Alloc(size_t count) {
 }  // namespace
 
 RankerState::~RankerState() {
-  void* ptrs[] = {params, grad, m, v, w1t, w2t, norm, moments, idx, pred, h1, h2, xn, dy, dh1, dh2, loss};
+  void* ptrs[] = {params, grad, m, v, w1t, w2t, norm, moments, idx, pred, h1, h2, xn, dy, dh1, dh2, loss, part, dpart};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
 }
@@ -232,10 +257,11 @@ void RankerState::Allocate(int batch_) {
     w2t = Alloc<unsigned short>(kHid * kHid);
     norm = Alloc<float>(kNormCount);
     moments = Alloc<double>(2 * kFeat + 2);
+    dpart = Alloc<double>(static_cast<size_t>(kMomentBlocks) * (2 * kFeat + 2));
     loss = Alloc<float>(1);
   }
   if (batch_ > batch) {
-    void* ptrs[] = {idx, pred, h1, h2, xn, dy, dh1, dh2};
+    void* ptrs[] = {idx, pred, h1, h2, xn, dy, dh1, dh2, part};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
     batch = batch_;
@@ -247,6 +273,8 @@ void RankerState::Allocate(int batch_) {
     h2 = Alloc<float>(static_cast<size_t>(batch) * kHid);
     dh1 = Alloc<float>(static_cast<size_t>(batch) * kHid);
     dh2 = Alloc<float>(static_cast<size_t>(batch) * kHid);
+    const size_t chunks = (static_cast<size_t>(batch) + kGemmChunk - 1) / kGemmChunk;
+    part = Alloc<float>(std::max<size_t>(chunks * kHid * kHid, static_cast<size_t>(kRedBlocks) * (kHid + 1)));
   }
 }
 
@@ -295,8 +323,8 @@ RankerView RankerState::View() const { return RankerView{w1t, w2t, params, norm}
 void RankerState::Normalize(const float* feats, const float* target, int n, hipStream_t stream) {
   Allocate(0);
   if (n <= 0) return;
-  HIP_CHECK(hipMemsetAsync(moments, 0, sizeof(double) * (2 * kFeat + 2), stream));
-  MomentsKernel<<<256, 256, 0, stream>>>(feats, target, n, moments);
+  MomentsKernel<<<kMomentBlocks, 256, 0, stream>>>(feats, target, n, dpart);
+  ReduceRowsKernel<double><<<1, 64, 0, stream>>>(dpart, kMomentBlocks, 2 * kFeat + 2, moments);
   NormFinalizeKernel<<<1, 64, 0, stream>>>(moments, n, norm);
 }
 
@@ -314,13 +342,19 @@ void RankerState::Gradient(const float* feats, const float* target, const int32_
   HIP_CHECK(hipMemsetAsync(loss, 0, sizeof(float), stream));
   Forward(feats, rows, n, pred, true, stream);
   LossGradKernel<<<(n + 255) / 256, 256, 0, stream>>>(pred, target, rows, norm, n, dy, loss);
-  OutputGradKernel<<<std::min(1024, n), kHid, 0, stream>>>(dy, h2, params, n, dh2, grad);
+  const int rblocks = std::min(kRedBlocks, n);
+  OutputGradKernel<<<rblocks, kHid, 0, stream>>>(dy, h2, params, n, dh2, part);
+  ReduceRowsKernel<float><<<1, kHid + 1, 0, stream>>>(part, rblocks, kHid + 1, grad + kOffW3);
   HiddenGradKernel<<<(n + 31) / 32, 256, 0, stream>>>(dh2, params, h1, n, dh1);
   const int chunks = (n + kGemmChunk - 1) / kGemmChunk;
-  GemmAtBKernel<<<dim3(chunks, (kHid / 32) * (kHid / 32)), 256, 0, stream>>>(h1, kHid, dh2, n, grad + kOffW2);
-  GemmAtBKernel<<<dim3(chunks, 1 * (kHid / 32)), 256, 0, stream>>>(xn, kFeat, dh1, n, grad + kOffW1);
-  ColSumKernel<<<std::min(1024, n), kHid, 0, stream>>>(dh1, n, grad + kOffB1);
-  ColSumKernel<<<std::min(1024, n), kHid, 0, stream>>>(dh2, n, grad + kOffB2);
+  GemmAtBKernel<<<dim3(chunks, (kHid / 32) * (kHid / 32)), 256, 0, stream>>>(h1, kHid, dh2, n, part);
+  ReduceRowsKernel<float><<<kHid * kHid / 256, 256, 0, stream>>>(part, chunks, kHid * kHid, grad + kOffW2);
+  GemmAtBKernel<<<dim3(chunks, 1 * (kHid / 32)), 256, 0, stream>>>(xn, kFeat, dh1, n, part);
+  ReduceRowsKernel<float><<<kFeat * kHid / 256, 256, 0, stream>>>(part, chunks, kFeat * kHid, grad + kOffW1);
+  ColSumKernel<<<rblocks, kHid, 0, stream>>>(dh1, n, part);
+  ReduceRowsKernel<float><<<1, kHid, 0, stream>>>(part, rblocks, kHid, grad + kOffB1);
+  ColSumKernel<<<rblocks, kHid, 0, stream>>>(dh2, n, part);
+  ReduceRowsKernel<float><<<1, kHid, 0, stream>>>(part, rblocks, kHid, grad + kOffB2);
 }
 
 void RankerState::AdamStep(float lr, hipStream_t stream) {

@@ -31,6 +31,8 @@ struct RankerState {
   float* dh1 = nullptr;
   float* dh2 = nullptr;
   float* loss = nullptr;
+  float* part = nullptr;    // per-block partial sums of the batch reductions
+  double* dpart = nullptr;  // per-block partial moments
   int step = 0;
 
   RankerState() = default;

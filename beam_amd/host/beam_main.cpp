@@ -217,7 +217,7 @@ int main(int argc, char** argv) {
               << "CUDA scoring build: " << beam_hip_build_mode() << "\n"
               << (c.write_output ? "Output directory: " + c.out_dir : std::string("Output writing: disabled")) << "\n";
 
-    beam_hip_options o{c.top_k, c.group_batch, c.hash_dims, c.max_group, 1, 0, c.seed};
+    beam_hip_options o{c.top_k, c.group_batch, c.hash_dims, c.max_group, 1, 0, c.seed, 0, 5, 0.25f, 0.9f};
     beam_hip_engine* e = nullptr;
     Check(beam_hip_create(reinterpret_cast<const int64_t*>(g.row_ptr.data()), g.col.data(), g.n, g.arcs, &o, &e),
           "beam_hip_create");

@@ -43,6 +43,18 @@ typedef struct beam_hip_options {
   int32_t ea_use_threshold; /* --ea-use-threshold (must be 1: reciprocal threshold) */
   int32_t device;           /* HIP device ordinal */
   uint64_t seed;            /* --seed */
+  /* Learned pair ranker (no reference counterpart; 0 = plain exact scoring).
+   * Iterations 1..ranker_train_iters score every candidate and sample
+   * (features, exact gain); the network is then trained on the GPU; later
+   * iterations score exactly only each group's top ranker_keep fraction of
+   * candidates (groups with at most 8 candidates keep all).  Each pruned
+   * iteration first audits a sample of groups; if the pruned set misses the
+   * best exact-gain pair in more than 1 - ranker_recall_min of them, that
+   * iteration and all later ones run plain. */
+  int32_t ranker;             /* 1 = on */
+  int32_t ranker_train_iters; /* default 5 */
+  float ranker_keep;          /* default 0.25 */
+  float ranker_recall_min;    /* default 0.9 */
 } beam_hip_options;
 
 typedef struct beam_hip_engine beam_hip_engine;
@@ -228,6 +240,21 @@ int beam_hip_ranker_adam(float* params, const float* grad, float* m, float* v, i
  * full batches; params and norm are updated in place, *loss = last batch's. */
 int beam_hip_ranker_train(float* params, float* norm, const float* feats, const float* target, int32_t n,
                           int32_t epochs, int32_t batch, float lr, uint64_t seed, float* loss);
+
+/* Ranker figures of the last run (engine created with ranker = 1): out[0]
+ * ranker on, [1] network ready, [2] training samples, [3] final training
+ * loss, [4] training ms, [5] audited groups where pruning applied and a
+ * positive best gain existed, [6] of those, groups whose best pair was kept,
+ * [7] iteration of the fallback to plain scoring (0: none), [8] pruned
+ * iterations, [9] candidate pairs generated, [10] pairs scored exactly.
+ * out needs 11 entries. */
+int beam_hip_ranker_stats(beam_hip_engine* engine, double* out);
+/* Copies the engine's network out (params, norm as for the entry points
+ * above), e.g. to reuse a network trained on another graph. */
+int beam_hip_ranker_get(beam_hip_engine* engine, float* params, float* norm);
+/* Installs a trained network: the engine then skips sampling and training
+ * (iterations 1..ranker_train_iters run plain, later ones prune). */
+int beam_hip_ranker_set(beam_hip_engine* engine, const float* params, const float* norm);
 
 /* ---- Graph input (host side) ---- */
 

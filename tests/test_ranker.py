@@ -127,3 +127,70 @@ def test_training_fits_a_smooth_target():
     before = mlp.loss(params, tnorm, feats, target)
     after = mlp.loss(trained, tnorm, feats, target)
     assert after < 0.2 * before and after < 0.2, (before, after, last)
+
+
+def _golden(name):
+    import json
+    with open(os.path.join(ROOT, "tests", "golden", f"{name}_summary.json")) as f:
+        return json.load(f)["metrics"]
+
+
+@pytest.mark.gpu
+def test_forced_fallback_is_plain(graph_path):
+    """An unreachable recall threshold makes the first audit fall back, so the
+    run is the plain method: bit-identical to the reference."""
+    m = _golden("am")
+    g = beam_amd.load_graph(graph_path("am"))
+    e = beam_amd.Engine(g, seed=2, ranker=True, ranker_recall_min=1.5)
+    e.run(20)
+    st = e.ranker_stats()
+    assert st["ready"] == 1 and st["samples"] > 1024
+    assert st["fallback_iter"] == 6 and st["pruned_iters"] == 0
+    assert beam_amd.partition_hash(e.labels()) == int(m["partition_hash"])
+    assert e.stats()["merge_exact_gain_calls"] == int(m["merge_exact_gain_calls"])
+
+
+@pytest.mark.gpu
+def test_ranker_path_is_lossless_and_close_to_plain(graph_path):
+    """Pruned scoring changes which merges are found but every commit is
+    costed exactly: the encoding stays consistent (P + Cp + Cm equals the
+    exact partition cost), fewer pairs are scored, and with the default
+    recall audit the cost ratio stays within 0.5 % of the plain method's."""
+    m = _golden("am")
+    g = beam_amd.load_graph(graph_path("am"))
+    e = beam_amd.Engine(g, seed=2, ranker=True)
+    e.run(20)
+    st = e.ranker_stats()
+    assert st["pruned_iters"] >= 1
+    assert st["scored_pairs"] < st["candidate_pairs"]
+    enc = e.encoding_counts()
+    assert enc["P"] + enc["Cp"] + enc["Cm"] == enc["exact_partition_cost"]
+    ratio = 2 * (enc["P"] + enc["Cp"] + enc["Cm"]) / (2.0 * int(m["input_edges_raw"]))
+    assert abs(ratio - float(m["cost_ratio_mags_compatible"])) <= 0.005 * float(m["cost_ratio_mags_compatible"])
+    # Deterministic: the same run again gives the same partition.
+    first = beam_amd.partition_hash(e.labels())
+    e.reset()
+    e.run(20)
+    assert beam_amd.partition_hash(e.labels()) == first
+
+
+@pytest.mark.gpu
+def test_trained_network_beats_untrained(graph_path):
+    """What training buys: the audited top-gain recall of the trained network
+    against the same pruning with the untrained (initial) network."""
+    g = beam_amd.load_graph(graph_path("am"))
+    trained = beam_amd.Engine(g, seed=2, ranker=True, ranker_recall_min=0.0)
+    trained.run(8)
+    t = trained.ranker_stats()
+    untrained = beam_amd.Engine(g, seed=2, ranker=True, ranker_recall_min=0.0)
+    params, norm = untrained.ranker_get()  # initial weights, unit normalisation
+    # Normalisation of the trained run with the untrained weights: only the
+    # learning differs.
+    _, tnorm = trained.ranker_get()
+    untrained.ranker_set(params, tnorm)
+    untrained.run(8)
+    u = untrained.ranker_stats()
+    assert t["audited"] >= 64 and u["audited"] >= 64
+    recall_t = t["audit_hits"] / t["audited"]
+    recall_u = u["audit_hits"] / u["audited"]
+    assert recall_t > recall_u + 0.1, (recall_t, recall_u)
