@@ -84,6 +84,50 @@ def cpu_baseline(name: str, seed: int, iterations: int) -> dict | None:
     }
 
 
+def ranker_leg(beam_amd, graph, seed: int, args, plain_enc: dict) -> dict:
+    """The learned-ranker method on the same workload and GPU: sampling in
+    iterations 1-5, training on the GPU, audited pruning to each group's top
+    25 % of candidates afterwards.  Throughput counts the candidate pairs the
+    legacy generator produced (before pruning) per second of 20-iteration
+    wall time, like the plain value."""
+    import time
+    import torch
+
+    engine = beam_amd.Engine(graph, seed=seed, ranker=True)
+    for _ in range(args.warmup):
+        engine.reset()
+        engine.run(args.iterations)
+    torch.cuda.synchronize()
+    total, cand = 0.0, 0
+    for _ in range(args.steps):
+        engine.reset()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        engine.run(args.iterations)
+        torch.cuda.synchronize()
+        total += time.perf_counter() - t0
+        cand += engine.ranker_stats()["candidate_pairs"]
+    st = engine.ranker_stats()
+    enc = engine.encoding_counts()
+    cost = 2 * (enc["P"] + enc["Cp"] + enc["Cm"]) / (2.0 * graph.input_edges_raw)
+    plain_cost = 2 * (plain_enc["P"] + plain_enc["Cp"] + plain_enc["Cm"]) / (2.0 * graph.input_edges_raw)
+    return {
+        "value": round(cand / total, 1),
+        "unit": "pairs/s",
+        "ms_per_step": round(total * 1e3 / args.steps, 3),
+        "cost_ratio_mags_compatible": round(cost, 6),
+        "cost_ratio_delta_rel": round((cost - plain_cost) / plain_cost, 6),
+        "lossless": enc["P"] + enc["Cp"] + enc["Cm"] == enc["exact_partition_cost"],
+        "pairs_scored_exactly": st["scored_pairs"],
+        "candidate_pairs": st["candidate_pairs"],
+        "training_samples": st["samples"],
+        "training_ms": round(st["train_ms"], 3),
+        "audit_top_gain_recall": round(st["audit_hits"] / st["audited"], 4) if st["audited"] else None,
+        "pruned_iterations": st["pruned_iters"],
+        "fallback_iteration": st["fallback_iter"] or None,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,6 +137,7 @@ def main() -> int:
     ap.add_argument("--iterations", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--no-ranker", action="store_true", help="skip the learned-ranker leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -166,6 +211,8 @@ def main() -> int:
             "groups_scored": kp["groups_scored"],
         }
 
+    ranker = None if args.no_ranker else ranker_leg(beam_amd, graph, seed, args, enc)
+
     value = pairs_total / total_s
     line = {
         "metric": METRIC,
@@ -196,6 +243,7 @@ def main() -> int:
         "cost_ratio_mags_compatible": round(2 * (enc["P"] + enc["Cp"] + enc["Cm"]) / (2.0 * graph.input_edges_raw), 6),
         "bit_identical_to_reference": identical,
         "roofline": roofline,
+        "ranker": ranker,
         "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(args.workload, seed, args.iterations),
     }
     print(json.dumps(line))
