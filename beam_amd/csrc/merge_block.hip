@@ -44,6 +44,7 @@ constexpr int kRegionBytes = ((kCandPhaseBytes > kTableOffset + kTableBytes ? kC
 // row's keys staged in the wave's join table (two 64-bit words per key).
 constexpr int kRankMax = kTableSlots * 8 / 16;
 static_assert(kSmallC * 8 <= kTableBytes, "small groups' survivor keys reuse the join tables");
+static_assert(kMaxGroup * (8 + 8 + 4) + kSmallE * 2 <= kTableBytes, "per-member scoring sums fit the join tables");
 static_assert(kMaxGroup * 16 * 8 <= kRegionBytes, "large groups' survivor keys fit the LDS region");
 
 struct GroupArrays {
@@ -62,6 +63,7 @@ struct GroupArrays {
   long long* cand_before;
   uint32_t* pass;       // survivor flags, then their prefix (C + 1)
   uint32_t* sig;        // ranker: 64 one-byte min-hash bins per member (16 words)
+  uint32_t* spec;       // large groups: special entries of each member (scoring)
   int32_t* tables;      // per-wave join tables (always LDS)
   // Global-mode companions of the int16 LDS arrays.
   int32_t* g_ent_i;
@@ -123,72 +125,6 @@ __device__ __forceinline__ long long ProxyOf(long long w) {
   return static_cast<long long>(static_cast<unsigned long long>(w) & 0xffffffffULL);
 }
 __device__ __forceinline__ long long CorrOf(long long w) { return w >> 32; }
-
-template <int kLanes>
-__device__ __forceinline__ int64_t LaneSum64(int64_t v) {
-#pragma unroll
-  for (int off = kLanes / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
-  return v;
-}
-
-// Exact MAGS-compatible gain of members (a, b) from the staged rows on
-// kLanes lanes (quotient_graph.cpp:340-462).  The merge-join sums separate
-// into per-row streams plus a correction over common neighbours:
-//   before = sum_{x in A} BC(sa,sx,ea) + sum_{x in B} BC(sb,sx,eb) + internal terms
-//   after  = sum_{x in A} BC(sw,sx,ea) + sum_{x in B} BC(sw,sx,eb)
-//          + sum_{x in A and B} [BC(sw,sx,ea+eb) - BC(sw,sx,ea) - BC(sw,sx,eb)] + internal
-// (x ranges over neighbours other than a and b).  The common-neighbour sum is
-// accumulated for every pair of members while the candidates are built (each
-// common neighbour is a target bucket both rows fall into), so a pair costs
-// one pass over its two staged rows.  Each staged entry carries its target's
-// size, so no global memory is read.
-template <int kLanes>
-__device__ void ScorePairLanes(const GroupArrays& A, const int32_t* m_rep, const int32_t* m_size,
-                               const int32_t* m_len, const int32_t* m_int, const uint32_t* m_eoff, int a, int b,
-                               long long corr, bool active, int* err, long long* out_gain, long long* out_before) {
-  const int sl_lane = LaneId() & (kLanes - 1);
-  int64_t before = 0, after = 0, eab_a = 0, eab_b = 0;
-  if (active) {
-    const int la = m_len[a], lb = m_len[b];
-    const uint32_t oa = m_eoff[a], ob = m_eoff[b];
-    const int rep_a = m_rep[a], rep_b = m_rep[b];
-    const int64_t sa = m_size[a], sb = m_size[b], sw = sa + sb;
-    for (int k = sl_lane; k < la + lb; k += kLanes) {
-      const bool in_a = k < la;
-      const uint32_t idx = in_a ? oa + k : ob + (k - la);
-      const int t = A.ent_t[idx];
-      const int64_t e = A.ent_e[idx];
-      if (t == (in_a ? rep_b : rep_a)) {
-        if (in_a) eab_a = e; else eab_b = e;
-        continue;
-      }
-      const int64_t st = A.ent_st[idx];
-      before += BlockCost(in_a ? sa : sb, st, e, false, err);
-      after += BlockCost(sw, st, e, false, err);
-    }
-  }
-  before = LaneSum64<kLanes>(before);
-  after = LaneSum64<kLanes>(after);
-  eab_a = LaneSum64<kLanes>(eab_a);
-  eab_b = LaneSum64<kLanes>(eab_b);
-  if (active && sl_lane == 0) {
-    if (eab_a != eab_b) atomicOr(err, kErrAsymmetric);
-    const int64_t sa = m_size[a], sb = m_size[b], ia = m_int[a], ib = m_int[b], sw = sa + sb;
-    before += BlockCost(sa, sa, ia, true, err) + BlockCost(sb, sb, ib, true, err) +
-              BlockCost(sa, sb, eab_a, false, err);
-    after += corr + BlockCost(sw, sw, ia + ib + eab_a, true, err);
-    *out_gain = before - after;
-    *out_before = before;
-  }
-}
-
-constexpr int kSeg = 16;  // lanes per pair in the short-pair scorer
-
-// True when the pair's two rows have at most 64 entries together, so a
-// 16-lane segment streams at most four entries per lane.
-__device__ __forceinline__ bool SmallPair(const int32_t* m_len, int a, int b) {
-  return m_len[a] + m_len[b] <= 64;
-}
 
 // What the ranker's features read from a staged group.
 struct FeatureSource {
@@ -372,12 +308,13 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
     A.cand_b = reinterpret_cast<uint16_t*>(c);
     A.g_ent_i = A.g_ent_rank = A.g_bucket = nullptr;
     A.sig = s_sig;
+    A.spec = nullptr;
   } else {
     if (tid == 0) {
       const unsigned long long e64 = E, tc64 = TC, g64 = static_cast<unsigned long long>(g), c64 = cmax;
       const unsigned long long bytes = e64 * 4 * 7 + tc64 * 4 + (tc64 + 1) * 8 + 16 + g64 * g64 * 8 +
                                        (g64 * g64 / 32 + 1) * 4 + 16 + c64 * (8 + 8 + 2 + 2) +
-                                       (c64 + 1) * 4 + g64 * 64 + 64;
+                                       (c64 + 1) * 4 + g64 * 64 + e64 * 4 + 64;
       const unsigned long long base = atomicAdd(p.scratch_top, (bytes + 255) & ~255ULL);
       s_base = base;
       s_flag = base + bytes > p.scratch_cap ? 1 : 0;
@@ -409,7 +346,8 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
     A.cand_a = reinterpret_cast<uint16_t*>(r); r += cmax * 2ULL;
     A.cand_b = reinterpret_cast<uint16_t*>(r); r += cmax * 2ULL;
     r = align8(r);
-    A.sig = reinterpret_cast<uint32_t*>(r);
+    A.sig = reinterpret_cast<uint32_t*>(r); r += static_cast<unsigned long long>(g) * 64;
+    A.spec = reinterpret_cast<uint32_t*>(r);
   }
   int16_t* s_ent_rank = reinterpret_cast<int16_t*>(s_region + kSmallE * 4);
   int16_t* s_bucket = s_ent_rank + kSmallE;
@@ -890,33 +828,96 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
   }
 
   stamp(10);
-  // Phase 9: exact gain of every candidate from the staged rows: pairs of
-  // short rows on 16-lane segments (four per wave), the rest one wave each.
+  // Phase 9: exact gain of every candidate (quotient_graph.cpp:340-462).
+  // With sw = sa + sb, a cross entry (x, e) of member a with 2e <= 1 + sa sx
+  // (sparse at sa) has BC(sa,sx,e) = BC(sw,sx,e) = e, so its before and after
+  // terms are the same for every partner; only entries that are dense at sa
+  // and entries pointing at other members (one of which may be the partner)
+  // need the pair.  Per member: S = sum of e over sparse non-member entries,
+  // D = sum of BC(sa,sx,e) over dense non-member entries, and the list of the
+  // special (dense or member) entries.  Then for the pair
+  //   before = S_a + D_a + S_b + D_b + sum_{member x != b, a} BC(s_side,sx,e) + internal terms
+  //   after  = S_a + S_b + sum_{special x != b, a} BC(sw,sx,e) + correction + internal
+  // with the common-neighbour correction from the candidate phase.
   {
-    const int seg = LaneId() / kSeg;
-    for (uint32_t base = static_cast<uint32_t>(WaveId()) * 4; base < C; base += kWaves * 4) {
-      const uint32_t pi = base + static_cast<uint32_t>(seg);
-      const bool in_range = pi < C;
-      const int a = in_range ? A.cand_a[pi] : 0, b = in_range ? A.cand_b[pi] : 0;
-      const bool mine = in_range && SmallPair(m_len, a, b);
-      const long long corr = mine ? CorrOf(A.proxy[a * g + b]) : 0;
-      long long gain = 0, before = 0;
-      ScorePairLanes<kSeg>(A, m_rep, m_size, m_len, m_int, m_eoff, a, b, corr, mine, p.err, &gain, &before);
-      if (mine && (LaneId() & (kSeg - 1)) == 0) {
-        A.cand_gain[pi] = gain;
-        A.cand_before[pi] = before;
+    long long* m_ssp = reinterpret_cast<long long*>(A.tables);  // the join tables are free here
+    long long* m_bden = m_ssp + kMaxGroup;
+    uint32_t* m_nspec = reinterpret_cast<uint32_t*>(m_bden + kMaxGroup);
+    uint16_t* spec16 = reinterpret_cast<uint16_t*>(m_nspec + kMaxGroup);
+    constexpr uint32_t kMemberFlag16 = 0x8000U, kMemberFlag32 = 0x80000000U;
+    for (int i = tid; i < g; i += kThreads) {
+      m_ssp[i] = 0;
+      m_bden[i] = 0;
+      m_nspec[i] = 0;
+    }
+    __syncthreads();
+    for (uint32_t e = tid; e < E; e += kThreads) {
+      if (A.ent_slot[e] < 0) continue;  // the doubled self entry is not a cross entry
+      const int i = small ? s_ent_i[e] : A.g_ent_i[e];
+      const int t = A.ent_t[e];
+      const int64_t ee = A.ent_e[e], st = A.ent_st[e], sa = m_size[i];
+      uint32_t h = HashInt(static_cast<uint32_t>(t)) & (kMemberSlots - 1);
+      bool member = false;
+      while (true) {
+        const int key = mh_key[h];
+        if (key == t) {
+          member = true;
+          break;
+        }
+        if (key == -1) break;
+        h = (h + 1) & (kMemberSlots - 1);
+      }
+      if (member || 2 * ee > 1 + sa * st) {
+        const uint32_t pos = m_eoff[i] + atomicAdd(&m_nspec[i], 1U);
+        if (small) {
+          spec16[pos] = static_cast<uint16_t>(e | (member ? kMemberFlag16 : 0U));
+        } else {
+          A.spec[pos] = e | (member ? kMemberFlag32 : 0U);
+        }
+        if (!member) {
+          atomicAdd(reinterpret_cast<unsigned long long*>(&m_bden[i]),
+                    static_cast<unsigned long long>(BlockCost(sa, st, ee, false, p.err)));
+        }
+      } else {
+        atomicAdd(reinterpret_cast<unsigned long long*>(&m_ssp[i]), static_cast<unsigned long long>(ee));
       }
     }
-    for (uint32_t pi = static_cast<uint32_t>(WaveId()); pi < C; pi += kWaves) {
+    __syncthreads();
+    for (uint32_t pi = tid; pi < C; pi += kThreads) {
       const int a = A.cand_a[pi], b = A.cand_b[pi];
-      if (SmallPair(m_len, a, b)) continue;
-      const long long corr = CorrOf(A.proxy[a * g + b]);
-      long long gain = 0, before = 0;
-      ScorePairLanes<kWave>(A, m_rep, m_size, m_len, m_int, m_eoff, a, b, corr, true, p.err, &gain, &before);
-      if (LaneId() == 0) {
-        A.cand_gain[pi] = gain;
-        A.cand_before[pi] = before;
+      const int64_t sa = m_size[a], sb = m_size[b], sw = sa + sb;
+      int64_t before = m_ssp[a] + m_bden[a] + m_ssp[b] + m_bden[b];
+      int64_t after = m_ssp[a] + m_ssp[b];
+      int64_t eab[2] = {0, 0};
+#pragma unroll
+      for (int side = 0; side < 2; ++side) {
+        const int me = side == 0 ? a : b;
+        const int other_rep = m_rep[side == 0 ? b : a];
+        const int64_t s_me = side == 0 ? sa : sb;
+        const uint32_t base = m_eoff[me], cnt = m_nspec[me];
+        for (uint32_t k = 0; k < cnt; ++k) {
+          const uint32_t v = small ? spec16[base + k] : A.spec[base + k];
+          const bool member = small ? (v & kMemberFlag16) != 0 : (v & kMemberFlag32) != 0;
+          const uint32_t e = small ? (v & ~kMemberFlag16) : (v & ~kMemberFlag32);
+          const int t = A.ent_t[e];
+          const int64_t ee = A.ent_e[e], st = A.ent_st[e];
+          if (member) {
+            if (t == other_rep) {
+              eab[side] = ee;
+              continue;
+            }
+            before += BlockCost(s_me, st, ee, false, p.err);
+          }
+          after += BlockCost(sw, st, ee, false, p.err);
+        }
       }
+      if (eab[0] != eab[1]) atomicOr(p.err, kErrAsymmetric);
+      const int64_t ia = m_int[a], ib = m_int[b];
+      before += BlockCost(sa, sa, ia, true, p.err) + BlockCost(sb, sb, ib, true, p.err) +
+                BlockCost(sa, sb, eab[0], false, p.err);
+      after += CorrOf(A.proxy[a * g + b]) + BlockCost(sw, sw, ia + ib + eab[0], true, p.err);
+      A.cand_gain[pi] = before - after;
+      A.cand_before[pi] = before;
     }
   }
   __syncthreads();
