@@ -953,7 +953,11 @@ void Engine::LaunchRound(int G, int nblocks, int block_size, double theta, Count
     }
     HIP_CHECK(hipEventRecord(ev_pool[ev_used++], stream));
   }
-  MergeGroupsKernel<<<score_grid, 512, 0, stream>>>(mp);
+  if (ranker_mode_now != kRankOff) {
+    MergeGroupsRankerKernel<<<score_grid, 512, 0, stream>>>(mp);
+  } else {
+    MergeGroupsKernel<<<score_grid, 512, 0, stream>>>(mp);
+  }
   if (profile) HIP_CHECK(hipEventRecord(ev_pool[ev_used++], stream));
 
   RoundParams rp{};
@@ -1113,7 +1117,7 @@ void Engine::RankerSetupIteration(int iter, int G, double theta) {
   mp.todo = audit_list;
   ranker_mode_now = kRankAudit;
   SetRankerParams(mp);
-  MergeGroupsKernel<<<score_grid, 512, 0, stream>>>(mp);
+  MergeGroupsRankerKernel<<<score_grid, 512, 0, stream>>>(mp);
   std::vector<long long> out(static_cast<size_t>(3 * count));
   HIP_CHECK(hipMemcpyAsync(out.data(), audit_out, sizeof(long long) * 3 * count, hipMemcpyDeviceToHost, stream));
   Sync();

@@ -193,7 +193,8 @@ struct CommitParams {
   int* err;
 };
 
-__global__ void MergeGroupsKernel(MergeParams p);
+__global__ void MergeGroupsKernel(MergeParams p);        // plain exact scoring
+__global__ void MergeGroupsRankerKernel(MergeParams p);  // with the learned ranker's modes
 __global__ void CommitKernel(CommitParams p);
 __global__ void TouchRound(RoundParams p);
 __global__ void FindCommitBlocks(RoundParams p);

@@ -203,6 +203,7 @@ __device__ __forceinline__ void RankerPredict(const FeatureSource& fsrc, const R
 // Candidate generation, exact scoring and selection for group gidx; results
 // go to sel_out (pairs), sel_count[slot], slot_stats[slot].  Every early exit
 // is taken by the whole workgroup.
+template <bool kRanker>
 __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int slot, int32_t* sel_out, int item) {
   long long* trace = nullptr;
   const long long t_start = static_cast<long long>(wall_clock64());
@@ -729,7 +730,7 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
 
   // Learned ranker: features of a candidate pair from the staged group.
   const FeatureSource fsrc{m_size, m_len, m_int, m_aoc, A.proxy, A.sig, g, p.theta};
-  if (p.ranker_mode != kRankOff) {
+  if (kRanker && p.ranker_mode != kRankOff) {
     // One-permutation min-hash of every member's neighbours: 64 bins of
     // 8-bit minima (0xff: empty), packed four to a word.
     for (int k = tid; k < g * 16; k += kThreads) A.sig[k] = 0xffffffffU;
@@ -751,7 +752,7 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
     }
     __syncthreads();
   }
-  if ((p.ranker_mode == kRankPrune || p.ranker_mode == kRankAudit) && C > 0) {
+  if (kRanker && (p.ranker_mode == kRankPrune || p.ranker_mode == kRankAudit) && C > 0) {
     // Predicted gains of all candidates, 32 per wave on MFMA (held as floats
     // in cand_before until the exact scoring overwrites it).
     float* pred = reinterpret_cast<float*>(A.cand_before);
@@ -922,7 +923,7 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
   }
   __syncthreads();
   stamp(11);
-  if (p.ranker_mode == kRankAudit) {
+  if (kRanker && p.ranker_mode == kRankAudit) {
     // The best exact gain of the group (lowest candidate index among ties)
     // and whether the ranker kept that pair; no selection.
     if (tid == 0) s_best = 0;
@@ -942,7 +943,7 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
     __syncthreads();
     return;
   }
-  if (p.ranker_mode == kRankCollect && p.samples != nullptr) {
+  if (kRanker && p.ranker_mode == kRankCollect && p.samples != nullptr) {
     // Training samples: (features, slog exact gain) of a hashed subset of
     // the scored pairs.
     for (uint32_t pi = tid; pi < C; pi += kThreads) {
@@ -1132,7 +1133,10 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
 
 }  // namespace
 
-__global__ __launch_bounds__(kThreads) void MergeGroupsKernel(MergeParams p) {
+// The plain method's kernel has no ranker code (its register budget stays
+// that of exact scoring); the ranker's modes run the second instantiation.
+template <bool kRanker>
+__device__ __forceinline__ void MergeGroupsBody(const MergeParams& p) {
   // Window mode (round == nullptr): workgroup s scores group first_group + s
   // into slot s.  Round mode: workgroups take queued groups until the queue
   // is drained; results are kept per group.
@@ -1165,8 +1169,11 @@ __global__ __launch_bounds__(kThreads) void MergeGroupsKernel(MergeParams p) {
       if (gidx >= p.ngroups) break;
       sel_out = p.sel + 2LL * slot * kMaxSelect;
     }
-    ScoreGroup(p, gidx, slot, sel_out, item);
+    ScoreGroup<kRanker>(p, gidx, slot, sel_out, item);
   }
 }
+
+__global__ __launch_bounds__(kThreads) void MergeGroupsKernel(MergeParams p) { MergeGroupsBody<false>(p); }
+__global__ __launch_bounds__(kThreads) void MergeGroupsRankerKernel(MergeParams p) { MergeGroupsBody<true>(p); }
 
 }  // namespace beam

@@ -84,6 +84,20 @@ def cpu_baseline(name: str, seed: int, iterations: int) -> dict | None:
     }
 
 
+def measured_traffic(workload: str):
+    """HBM bytes per scoring launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE passes (calibrated against a copy kernel; see the file for the
+    commands), for the workload they were taken on; None otherwise."""
+    path = os.path.join(ROOT, "profiles", f"merge_{workload}_hbm_counters.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch="):
+                return int(line.split("=", 1)[1])
+    return None
+
+
 def ranker_leg(beam_amd, graph, seed: int, args, plain_enc: dict) -> dict:
     """The learned-ranker method on the same workload and GPU: sampling in
     iterations 1-5, training on the GPU, audited pruning to each group's top
@@ -138,6 +152,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--no-ranker", action="store_true", help="skip the learned-ranker leg")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one 256 MiB device-to-device copy kernel (reference point for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -146,6 +162,14 @@ def main() -> int:
 
     import beam_amd
     import torch
+
+    if args.copy_calibration:
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")  # 256 MiB
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     desc, seed = WORKLOADS[args.workload]
     graph = beam_amd.load_graph(graph_path(args.workload))
@@ -201,7 +225,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 6),
-            "traffic": None,
+            "traffic": measured_traffic(args.workload),
             "kernel": "beam::MergeGroupsKernel",
             "avg_launch_us": round(kp["group_kernel_ms"] * 1e3 / launches, 3),
             "launches": kp["group_kernel_launches"],
