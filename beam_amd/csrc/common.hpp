@@ -79,6 +79,18 @@ __device__ __forceinline__ uint64_t FallbackRank(uint64_t seed, int rep, int ite
   return SplitMix64(x);
 }
 
+// Values every lane of the wave holds alike (read from LDS after a barrier,
+// say), marked so the compiler keeps them and what derives from them in
+// scalar registers.
+__device__ __forceinline__ int Uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ uint32_t Uniform(uint32_t x) {
+  return static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(x)));
+}
+__device__ __forceinline__ unsigned long long Uniform(unsigned long long x) {
+  const uint32_t lo = Uniform(static_cast<uint32_t>(x)), hi = Uniform(static_cast<uint32_t>(x >> 32));
+  return (static_cast<unsigned long long>(hi) << 32) | lo;
+}
+
 __device__ __forceinline__ int LaneId() { return static_cast<int>(threadIdx.x & (kWave - 1)); }
 __device__ __forceinline__ int WaveId() { return static_cast<int>(threadIdx.x / kWave); }
 

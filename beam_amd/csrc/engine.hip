@@ -1190,9 +1190,11 @@ void Engine::Iterate(int iter) {
   tracing_now = trace_iter == iter;
   if (tracing_now) {
     if (trace_buf) (void)hipFree(trace_buf);
-    trace_buf = DevAlloc<long long>(16ULL * static_cast<size_t>(G) + 16);
-    HIP_CHECK(hipMemsetAsync(trace_buf, 0, sizeof(long long) * (16ULL * G + 16), stream));
-    trace_groups = G;
+    trace_groups = 4LL * G + 1024;  // record capacity (rescorings included)
+    trace_buf = DevAlloc<long long>(16ULL * static_cast<size_t>(trace_groups + 1));
+    HIP_CHECK(hipMemsetAsync(trace_buf, 0, sizeof(long long) * 16ULL * (trace_groups + 1), stream));
+    HIP_CHECK(hipMemcpyAsync(trace_buf + 1, &trace_groups, sizeof(long long), hipMemcpyHostToDevice, stream));
+    HIP_CHECK(hipStreamSynchronize(stream));
   }
   const int bs = opt.group_batch_size;
   const int nblocks = (G + bs - 1) / bs;
@@ -1467,9 +1469,13 @@ int64_t beam_hip_trace_fetch(beam_hip_engine* engine, int64_t* out, int64_t cap_
   int64_t count = -1;
   Guard([&] {
     auto& e = engine->e;
-    count = e.trace_buf ? e.trace_groups : 0;
+    count = 0;
+    if (e.trace_buf != nullptr) {
+      HIP_CHECK(hipMemcpy(&count, e.trace_buf, sizeof(long long), hipMemcpyDeviceToHost));
+      count = std::min<int64_t>(count, e.trace_groups);
+    }
     const int64_t take = std::min<int64_t>(count, cap_groups);
-    if (take > 0) HIP_CHECK(hipMemcpy(out, e.trace_buf, sizeof(long long) * 16 * take, hipMemcpyDeviceToHost));
+    if (take > 0) HIP_CHECK(hipMemcpy(out, e.trace_buf + 16, sizeof(long long) * 16 * take, hipMemcpyDeviceToHost));
     return 0;
   });
   return count;

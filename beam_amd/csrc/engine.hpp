@@ -92,8 +92,9 @@ struct MergeParams {
   int32_t* reset_queue;           // this window's neighbour queue length, zeroed here
   Counters* ctr;
   int* err;
-  // Optional phase trace: 16 values per group at trace[16 * group]
-  // (g, E, bucket pairs, candidates, survivors, then wall-clock stamps).
+  // Optional phase trace: record 0 = (count, capacity), then 16 values per
+  // scoring (g, E, bucket pairs, candidates, survivors, wall-clock stamps,
+  // group index).
   long long* trace;
   // Optional taps for tests: candidate pairs and (gain, before) per slot.
   int32_t* dbg_pairs;

@@ -200,11 +200,79 @@ __device__ __forceinline__ void RankerPredict(const FeatureSource& fsrc, const R
   }
 }
 
+// Index of the last offset <= value among off[lo..hi], given
+// off[lo] <= value < off[hi].
+__device__ __forceinline__ int UpperSegmentIn(const uint32_t* off, int lo, int hi, uint32_t value) {
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (off[mid] <= value) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// What the bucket-pair phase of a large group reads (staged arrays in the
+// group's scratch, members in LDS) and updates.
+struct LargeView {
+  const uint32_t* pair_pref;  // bucket pair offsets (tc + 1)
+  const uint32_t* th_cnt;     // bucket entry offsets (tc + 1)
+  const int32_t* bucket;      // staged entry of each bucket position
+  const int32_t* ent_i;       // member of each staged entry
+  const int32_t* ent_t;
+  const int32_t* ent_st;
+  const int32_t* ent_e;
+  const int32_t* rep;         // member reps
+  const int32_t* size;        // member sizes
+  long long* proxy;           // g x g proxy words
+  int g, tc;
+};
+
+// Bucket pairs [p0, p1) of a large group (sweg.cpp:1863-1903): for two
+// members sharing target t, delta = EC(a,t,ea) + EC(b,t,eb) - EC(a+b,t,ea+eb)
+// is accumulated when positive, and t's term of the pair's exact-gain
+// common-neighbour correction goes into the high word.  All threads of the
+// workgroup; the adds are integer atomics, so the order does not matter.
+__device__ __forceinline__ void ProxyPairs(const LargeView& v, uint32_t p0, uint32_t p1, int* err) {
+  if (p0 >= p1) return;
+  const int slo = UpperSegment(v.pair_pref, v.tc, p0);
+  const int shi = UpperSegment(v.pair_pref, v.tc, p1 - 1) + 1;
+  const int g = v.g;
+  for (uint32_t pi = p0 + threadIdx.x; pi < p1; pi += kThreads) {
+    const int sl = UpperSegmentIn(v.pair_pref, slo, shi, pi);
+    const uint32_t l = pi - v.pair_pref[sl];
+    const uint32_t off = v.th_cnt[sl];
+    const uint32_t sz = v.th_cnt[sl + 1] - off;
+    uint32_t x, y;
+    TriDecode(l, sz, &x, &y);
+    const int ex = v.bucket[off + x];
+    const int ey = v.bucket[off + y];
+    const int i = v.ent_i[ex];
+    const int j = v.ent_i[ey];
+    const int t = v.ent_t[ex];
+    const int64_t st = v.ent_st[ex];
+    const int64_t ea = v.ent_e[ex], eb = v.ent_e[ey];
+    const int64_t sa = v.size[i], sb = v.size[j];
+    const int ra = v.rep[i], rb = v.rep[j];
+    const int64_t old_cost = EncodeCost(ra, sa, t, st, ea) + EncodeCost(rb, sb, t, st, eb);
+    const int64_t new_cost = EncodeCost(ra, sa + sb, t, st, ea + eb);
+    const int64_t delta = old_cost - new_cost;
+    const int64_t sw = sa + sb;
+    const int64_t corr = BlockCost(sw, st, ea + eb, false, err) - BlockCost(sw, st, ea, false, err) -
+                         BlockCost(sw, st, eb, false, err);
+    const unsigned long long add =
+        (delta > 0 ? static_cast<unsigned long long>(delta) : 0ULL) + (static_cast<unsigned long long>(corr) << 32);
+    if (add != 0) {
+      const int lo = min(i, j), hi = max(i, j);
+      atomicAdd(reinterpret_cast<unsigned long long*>(&v.proxy[lo * g + hi]), add);
+    }
+  }
+}
+
 // Candidate generation, exact scoring and selection for group gidx; results
 // go to sel_out (pairs), sel_count[slot], slot_stats[slot].  Every early exit
 // is taken by the whole workgroup.
 template <bool kRanker>
-__device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int slot, int32_t* sel_out, int item) {
+__device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int slot, int32_t* sel_out, int item,
+                                           unsigned char* s_region) {
   long long* trace = nullptr;
   const long long t_start = static_cast<long long>(wall_clock64());
   auto stamp = [&](int k) {
@@ -215,17 +283,20 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
     p.sel_count[slot] = 0;
     p.slot_stats[static_cast<long long>(slot) * kSlotStats + kStatGroup] = 0;
   }
-  const int g = p.glen[gidx];
+  const int g = Uniform(p.glen[gidx]);
   if (g < 2) return;
   if (g > kMaxGroup) {
     if (tid == 0) atomicOr(p.err, kErrSelection);
     return;
   }
-  if (p.trace != nullptr) {
-    trace = p.trace + 16LL * gidx;
-    if (tid == 0) {
+  if (p.trace != nullptr && tid == 0) {
+    // Record 0 holds the record count and capacity; one record per scoring.
+    const unsigned long long r = atomicAdd(reinterpret_cast<unsigned long long*>(p.trace), 1ULL);
+    if (r < static_cast<unsigned long long>(p.trace[1])) {
+      trace = p.trace + 16LL * static_cast<long long>(r + 1);
       trace[0] = g;
       trace[5] = t_start;
+      trace[14] = gidx;
     }
   }
 
@@ -241,7 +312,6 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
   __shared__ uint32_t s_candbits[kSmallG * kSmallG / 32];
   __shared__ int32_t s_ent_t[kSmallE], s_ent_e[kSmallE], s_ent_st[kSmallE];
   __shared__ int16_t s_ent_i[kSmallE];
-  __shared__ __align__(16) unsigned char s_region[kRegionBytes];
   __shared__ unsigned char s_used[kMaxGroup];
   __shared__ unsigned long long s_chosen[kMaxSelect];
   __shared__ unsigned long long s_gain, s_work, s_entries;
@@ -277,8 +347,9 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
     m_eoff[i] = static_cast<uint32_t>(m_len[i] + (m_int[i] > 0 ? 1 : 0));
   }
   __syncthreads();
-  const uint32_t E = ScanInPlace(m_eoff, g, wave_tot);
+  const uint32_t E = Uniform(ScanInPlace(m_eoff, g, wave_tot));
   stamp(6);
+  const bool small = E <= kSmallE && g <= kSmallG;
 
   // Phase 2: storage -- LDS for small groups, the block scratch pool otherwise.
   uint32_t TC = 64;
@@ -286,7 +357,6 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
   const int topk = max(1, min(p.top_k, g - 1));  // sweg.cpp:1620
   const uint32_t cmax = min(static_cast<uint32_t>(g) * static_cast<uint32_t>(topk),
                             static_cast<uint32_t>(g) * static_cast<uint32_t>(g - 1) / 2);
-  const bool small = E <= kSmallE && g <= kSmallG;
   GroupArrays A;
   A.tables = reinterpret_cast<int32_t*>(s_region + kTableOffset);
   if (small) {
@@ -323,7 +393,7 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
     }
     __syncthreads();
     if (s_flag) return;
-    unsigned char* r = p.scratch + s_base;
+    unsigned char* r = p.scratch + Uniform(s_base);
     auto align8 = [](unsigned char* x) {
       return reinterpret_cast<unsigned char*>((reinterpret_cast<uintptr_t>(x) + 7) & ~uintptr_t(7));
     };
@@ -420,22 +490,23 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
     A.pair_pref[k] = s >= 2 ? s * (s - 1) / 2 : 0;
   }
   __syncthreads();
-  const uint32_t PT = ScanInPlace(A.pair_pref, static_cast<int>(TC), wave_tot);
+  const uint32_t PT = Uniform(ScanInPlace(A.pair_pref, static_cast<int>(TC), wave_tot));
 
   // Phase 5: bucket pair expansion (sweg.cpp:1863-1903): for two members
   // sharing target t, delta = EC(a,t,ea) + EC(b,t,eb) - EC(a+b,t,ea+eb),
   // accumulated only when positive.
-  for (uint32_t pi = tid; pi < PT; pi += kThreads) {
+  // Small groups here, from LDS; large groups through ProxyPairs below.
+  for (uint32_t pi = tid; small && pi < PT; pi += kThreads) {
     const int sl = UpperSegment(A.pair_pref, static_cast<int>(TC), pi);
     const uint32_t l = pi - A.pair_pref[sl];
     const uint32_t off = A.th_cnt[sl];
     const uint32_t s = A.th_cnt[sl + 1] - off;
     uint32_t x, y;
     TriDecode(l, s, &x, &y);
-    const int ex = small ? s_bucket[off + x] : A.g_bucket[off + x];
-    const int ey = small ? s_bucket[off + y] : A.g_bucket[off + y];
-    const int i = small ? s_ent_i[ex] : A.g_ent_i[ex];
-    const int j = small ? s_ent_i[ey] : A.g_ent_i[ey];
+    const int ex = s_bucket[off + x];
+    const int ey = s_bucket[off + y];
+    const int i = s_ent_i[ex];
+    const int j = s_ent_i[ey];
     const int t = A.ent_t[ex];
     const int64_t st = A.ent_st[ex];
     const int64_t ea = A.ent_e[ex], eb = A.ent_e[ey];
@@ -455,6 +526,11 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
       const int lo = min(i, j), hi = max(i, j);
       atomicAdd(reinterpret_cast<unsigned long long*>(&A.proxy[lo * g + hi]), add);
     }
+  }
+  if (!small) {
+    const LargeView lv{A.pair_pref, A.th_cnt, A.g_bucket, A.g_ent_i, A.ent_t, A.ent_st, A.ent_e,
+                       m_rep, m_size, A.proxy, g, static_cast<int>(TC)};
+    ProxyPairs(lv, 0, PT, p.err);
   }
   // Phase 6: direct edges between members (sweg.cpp:1905-1924).
   for (uint32_t e = tid; e < E; e += kThreads) {
@@ -713,7 +789,7 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
     m_rowcnt[lo] = cnt;
   }
   __syncthreads();
-  uint32_t C = ScanInPlace(m_rowcnt, g, wave_tot);
+  uint32_t C = Uniform(ScanInPlace(m_rowcnt, g, wave_tot));
   const uint32_t C0 = C;
   for (int lo = tid; lo < g; lo += kThreads) {
     uint32_t pos = m_rowcnt[lo];
@@ -1149,6 +1225,7 @@ __device__ __forceinline__ void MergeGroupsBody(const MergeParams& p) {
     if (slot == 0 && p.reset_queue != nullptr) *p.reset_queue = 0;
   }
   __shared__ int s_item;
+  __shared__ __align__(16) unsigned char s_region[kRegionBytes];
   const int ntodo = round ? p.round->ntodo : 1;
   for (int pass = 0; pass < ntodo; ++pass) {
     int gidx, slot, item;
@@ -1156,7 +1233,7 @@ __device__ __forceinline__ void MergeGroupsBody(const MergeParams& p) {
     if (round) {
       if (threadIdx.x == 0) s_item = atomicAdd(&p.round->steal, 1);
       __syncthreads();
-      item = s_item;
+      item = Uniform(s_item);
       __syncthreads();
       if (item >= ntodo) break;
       gidx = p.todo[item];
@@ -1169,7 +1246,7 @@ __device__ __forceinline__ void MergeGroupsBody(const MergeParams& p) {
       if (gidx >= p.ngroups) break;
       sel_out = p.sel + 2LL * slot * kMaxSelect;
     }
-    ScoreGroup<kRanker>(p, gidx, slot, sel_out, item);
+    ScoreGroup<kRanker>(p, gidx, slot, sel_out, item, s_region);
   }
 }
 

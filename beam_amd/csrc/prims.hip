@@ -142,9 +142,17 @@ __global__ void KeyBits(const SortRec* in, int n, unsigned long long* acc) {
     a &= in[i].key;
     s |= in[i].seg;
   }
-  atomicOr(&acc[0], o);
-  atomicAnd(&acc[1], a);
-  atomicOr(&acc[2], s);
+  // One atomic per wave.
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    o |= __shfl_xor(o, off, kWave);
+    a &= __shfl_xor(a, off, kWave);
+    s |= __shfl_xor(s, off, kWave);
+  }
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    atomicOr(&acc[0], o);
+    atomicAnd(&acc[1], a);
+    atomicOr(&acc[2], s);
+  }
 }
 
 }  // namespace

@@ -149,12 +149,13 @@ int beam_hip_profile(beam_hip_engine* engine, int32_t enable);
 int beam_hip_kernel_profile(beam_hip_engine* engine, double* out);
 
 /* Phase trace of the group kernel for one iteration (0 disables): for every
- * group of that iteration, 16 values: g, staged entries, bucket pairs,
- * candidates, survivors, then wall-clock stamps (100 MHz) at kernel entry and
+ * scoring of a group in that iteration (rescorings included, in start
+ * order, at most 4 G + 1024 records), 16 values: g, staged entries, bucket
+ * pairs, candidates, survivors, then wall-clock stamps (100 MHz) at entry and
  * after members, staging, proxy accumulation, top-k, candidate listing,
- * scoring, selection and queueing. */
+ * scoring, selection and queueing, then the group index and 0. */
 int beam_hip_trace(beam_hip_engine* engine, int32_t iter);
-/* Copies up to cap_groups traced groups; returns the group count traced. */
+/* Copies up to cap_groups trace records; returns the record count. */
 int64_t beam_hip_trace_fetch(beam_hip_engine* engine, int64_t* out, int64_t cap_groups);
 
 /* FNV-1a 64 over node labels (Sweg::PartitionHash, src/sweg.cpp:2920-2928). */
