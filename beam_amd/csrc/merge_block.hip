@@ -46,6 +46,13 @@ constexpr int kRankMax = kTableSlots * 8 / 16;
 static_assert(kSmallC * 8 <= kTableBytes, "small groups' survivor keys reuse the join tables");
 static_assert(kMaxGroup * (8 + 8 + 4) + kSmallE * 2 <= kTableBytes, "per-member scoring sums fit the join tables");
 static_assert(kMaxGroup * 16 * 8 <= kRegionBytes, "large groups' survivor keys fit the LDS region");
+// Large groups' bucket-pair phase keeps in the LDS region a block index
+// (first non-empty bucket of every block of 2^shift pairs, at most kBlkMax
+// blocks) and, when they fit, the compacted buckets' pair prefix.
+constexpr int kBlkMax = 4096;
+constexpr int kNzPrefOffset = ((kBlkMax + 1) * 4 + 15) & ~15;
+constexpr int kNzLds = (kRegionBytes - kNzPrefOffset) / 4 - 1;
+static_assert(kNzLds >= 8192, "the LDS region holds a useful compacted-bucket prefix");
 
 struct GroupArrays {
   int32_t* ent_t;       // target rep per staged entry
@@ -68,7 +75,14 @@ struct GroupArrays {
   // Global-mode companions of the int16 LDS arrays.
   int32_t* g_ent_i;
   int32_t* g_ent_rank;
-  int32_t* g_bucket;
+  int32_t* g_bucket;    // large: (member << 32 | count) per bucket position, 8 B
+  // Large groups' bucket-pair index: target size per slot, the slots with
+  // at least two entries compacted (position, {offset, size, target,
+  // target size}, pair prefix).
+  int32_t* th_st;
+  uint32_t* nz_pos;
+  uint4* nz_info;
+  uint32_t* nz_pref;
 };
 
 // Exclusive scan in place over n entries of `data` (generic pointer), with
@@ -78,9 +92,11 @@ __device__ uint32_t ScanInPlace(uint32_t* data, int n, uint32_t* wave_tot) {
   const int begin = static_cast<int>(threadIdx.x) * per;
   const int end = min(n, begin + per);
   uint32_t local = 0;
+#pragma unroll 8
   for (int i = begin; i < end; ++i) local += data[i];
   uint32_t total = 0;
   uint32_t run = BlockExclusiveScan(local, wave_tot, &total);
+#pragma unroll 8
   for (int i = begin; i < end; ++i) {
     const uint32_t v = data[i];
     data[i] = run;
@@ -200,73 +216,6 @@ __device__ __forceinline__ void RankerPredict(const FeatureSource& fsrc, const R
   }
 }
 
-// Index of the last offset <= value among off[lo..hi], given
-// off[lo] <= value < off[hi].
-__device__ __forceinline__ int UpperSegmentIn(const uint32_t* off, int lo, int hi, uint32_t value) {
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (off[mid] <= value) lo = mid; else hi = mid;
-  }
-  return lo;
-}
-
-// What the bucket-pair phase of a large group reads (staged arrays in the
-// group's scratch, members in LDS) and updates.
-struct LargeView {
-  const uint32_t* pair_pref;  // bucket pair offsets (tc + 1)
-  const uint32_t* th_cnt;     // bucket entry offsets (tc + 1)
-  const int32_t* bucket;      // staged entry of each bucket position
-  const int32_t* ent_i;       // member of each staged entry
-  const int32_t* ent_t;
-  const int32_t* ent_st;
-  const int32_t* ent_e;
-  const int32_t* rep;         // member reps
-  const int32_t* size;        // member sizes
-  long long* proxy;           // g x g proxy words
-  int g, tc;
-};
-
-// Bucket pairs [p0, p1) of a large group (sweg.cpp:1863-1903): for two
-// members sharing target t, delta = EC(a,t,ea) + EC(b,t,eb) - EC(a+b,t,ea+eb)
-// is accumulated when positive, and t's term of the pair's exact-gain
-// common-neighbour correction goes into the high word.  All threads of the
-// workgroup; the adds are integer atomics, so the order does not matter.
-__device__ __forceinline__ void ProxyPairs(const LargeView& v, uint32_t p0, uint32_t p1, int* err) {
-  if (p0 >= p1) return;
-  const int slo = UpperSegment(v.pair_pref, v.tc, p0);
-  const int shi = UpperSegment(v.pair_pref, v.tc, p1 - 1) + 1;
-  const int g = v.g;
-  for (uint32_t pi = p0 + threadIdx.x; pi < p1; pi += kThreads) {
-    const int sl = UpperSegmentIn(v.pair_pref, slo, shi, pi);
-    const uint32_t l = pi - v.pair_pref[sl];
-    const uint32_t off = v.th_cnt[sl];
-    const uint32_t sz = v.th_cnt[sl + 1] - off;
-    uint32_t x, y;
-    TriDecode(l, sz, &x, &y);
-    const int ex = v.bucket[off + x];
-    const int ey = v.bucket[off + y];
-    const int i = v.ent_i[ex];
-    const int j = v.ent_i[ey];
-    const int t = v.ent_t[ex];
-    const int64_t st = v.ent_st[ex];
-    const int64_t ea = v.ent_e[ex], eb = v.ent_e[ey];
-    const int64_t sa = v.size[i], sb = v.size[j];
-    const int ra = v.rep[i], rb = v.rep[j];
-    const int64_t old_cost = EncodeCost(ra, sa, t, st, ea) + EncodeCost(rb, sb, t, st, eb);
-    const int64_t new_cost = EncodeCost(ra, sa + sb, t, st, ea + eb);
-    const int64_t delta = old_cost - new_cost;
-    const int64_t sw = sa + sb;
-    const int64_t corr = BlockCost(sw, st, ea + eb, false, err) - BlockCost(sw, st, ea, false, err) -
-                         BlockCost(sw, st, eb, false, err);
-    const unsigned long long add =
-        (delta > 0 ? static_cast<unsigned long long>(delta) : 0ULL) + (static_cast<unsigned long long>(corr) << 32);
-    if (add != 0) {
-      const int lo = min(i, j), hi = max(i, j);
-      atomicAdd(reinterpret_cast<unsigned long long*>(&v.proxy[lo * g + hi]), add);
-    }
-  }
-}
-
 // Candidate generation, exact scoring and selection for group gidx; results
 // go to sel_out (pairs), sel_count[slot], slot_stats[slot].  Every early exit
 // is taken by the whole workgroup.
@@ -323,6 +272,7 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
   __shared__ uint32_t s_prefix, s_need;
   __shared__ int s_pruned;
   __shared__ unsigned long long s_best;
+  __shared__ int16_t s_topj[kWaves][kRankMax];
 
   // Phase 1: members, member lookup table, staged entry offsets.
   const int start = p.gstart[gidx];
@@ -377,13 +327,16 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
     A.pass = reinterpret_cast<uint32_t*>(c); c += (kSmallC + 1) * 4;
     A.cand_a = reinterpret_cast<uint16_t*>(c); c += kSmallC * 2;
     A.cand_b = reinterpret_cast<uint16_t*>(c);
-    A.g_ent_i = A.g_ent_rank = A.g_bucket = nullptr;
+    A.g_ent_i = A.g_ent_rank = A.g_bucket = A.th_st = nullptr;
+    A.nz_pos = A.nz_pref = nullptr;
+    A.nz_info = nullptr;
     A.sig = s_sig;
     A.spec = nullptr;
   } else {
     if (tid == 0) {
       const unsigned long long e64 = E, tc64 = TC, g64 = static_cast<unsigned long long>(g), c64 = cmax;
-      const unsigned long long bytes = e64 * 4 * 7 + tc64 * 4 + (tc64 + 1) * 8 + 16 + g64 * g64 * 8 +
+      const unsigned long long bytes = e64 * 4 * 8 + tc64 * 4 + (tc64 + 1) * 8 + tc64 * 4 + (tc64 + 1) * 4 + 16 +
+                                       tc64 * 16 + (tc64 + 1) * 4 + 16 + g64 * g64 * 8 +
                                        (g64 * g64 / 32 + 1) * 4 + 16 + c64 * (8 + 8 + 2 + 2) +
                                        (c64 + 1) * 4 + g64 * 64 + e64 * 4 + 64;
       const unsigned long long base = atomicAdd(p.scratch_top, (bytes + 255) & ~255ULL);
@@ -403,10 +356,15 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
     A.ent_slot = reinterpret_cast<int32_t*>(r); r += E * 4ULL;
     A.g_ent_i = reinterpret_cast<int32_t*>(r); r += E * 4ULL;
     A.g_ent_rank = reinterpret_cast<int32_t*>(r); r += E * 4ULL;
-    A.g_bucket = reinterpret_cast<int32_t*>(r); r += E * 4ULL;
+    A.g_bucket = reinterpret_cast<int32_t*>(r); r += E * 8ULL;  // 8-aligned: 24 E bytes precede it
     A.th_key = reinterpret_cast<int32_t*>(r); r += TC * 4ULL;
     A.th_cnt = reinterpret_cast<uint32_t*>(r); r += (TC + 1) * 4ULL;
     A.pair_pref = reinterpret_cast<uint32_t*>(r); r += (TC + 1) * 4ULL;
+    A.th_st = reinterpret_cast<int32_t*>(r); r += TC * 4ULL;
+    A.nz_pos = reinterpret_cast<uint32_t*>(r); r += (TC + 1) * 4ULL;
+    r = reinterpret_cast<unsigned char*>((reinterpret_cast<uintptr_t>(r) + 15) & ~uintptr_t(15));
+    A.nz_info = reinterpret_cast<uint4*>(r); r += TC * 16ULL;
+    A.nz_pref = reinterpret_cast<uint32_t*>(r); r += (TC + 1) * 4ULL;
     r = align8(r);
     A.proxy = reinterpret_cast<long long*>(r); r += static_cast<unsigned long long>(g) * g * 8;
     A.candbits = reinterpret_cast<uint32_t*>(r); r += (static_cast<unsigned long long>(g) * g / 32 + 1) * 4;
@@ -469,6 +427,7 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
       }
       sl = static_cast<int>(h);
       rank = static_cast<int>(atomicAdd(&A.th_cnt[h], 1U));
+      if (!small) A.th_st[h] = st;  // every entry of target t writes t's size
     }
     A.ent_slot[e] = sl;
     if (small) s_ent_rank[e] = static_cast<int16_t>(rank); else A.g_ent_rank[e] = rank;
@@ -478,13 +437,21 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
   stamp(7);
   // Phase 4: bucket offsets, scatter, pairs per bucket.
   ScanInPlace(A.th_cnt, static_cast<int>(TC), wave_tot);
+#pragma unroll 4
   for (uint32_t e = tid; e < E; e += kThreads) {
     const int sl = A.ent_slot[e];
     if (sl < 0) continue;
     const int rank = small ? s_ent_rank[e] : A.g_ent_rank[e];
     const uint32_t pos = A.th_cnt[sl] + static_cast<uint32_t>(rank);
-    if (small) s_bucket[pos] = static_cast<int16_t>(e); else A.g_bucket[pos] = static_cast<int32_t>(e);
+    if (small) {
+      s_bucket[pos] = static_cast<int16_t>(e);
+    } else {
+      reinterpret_cast<unsigned long long*>(A.g_bucket)[pos] =
+          (static_cast<unsigned long long>(static_cast<uint32_t>(A.g_ent_i[e])) << 32) |
+          static_cast<uint32_t>(A.ent_e[e]);
+    }
   }
+#pragma unroll 8
   for (uint32_t k = tid; k < TC; k += kThreads) {
     const uint32_t s = A.th_cnt[k + 1] - A.th_cnt[k];
     A.pair_pref[k] = s >= 2 ? s * (s - 1) / 2 : 0;
@@ -495,7 +462,7 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
   // Phase 5: bucket pair expansion (sweg.cpp:1863-1903): for two members
   // sharing target t, delta = EC(a,t,ea) + EC(b,t,eb) - EC(a+b,t,ea+eb),
   // accumulated only when positive.
-  // Small groups here, from LDS; large groups through ProxyPairs below.
+  // Small groups here, from LDS; large groups below.
   for (uint32_t pi = tid; small && pi < PT; pi += kThreads) {
     const int sl = UpperSegment(A.pair_pref, static_cast<int>(TC), pi);
     const uint32_t l = pi - A.pair_pref[sl];
@@ -527,11 +494,89 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
       atomicAdd(reinterpret_cast<unsigned long long*>(&A.proxy[lo * g + hi]), add);
     }
   }
-  if (!small) {
-    const LargeView lv{A.pair_pref, A.th_cnt, A.g_bucket, A.g_ent_i, A.ent_t, A.ent_st, A.ent_e,
-                       m_rep, m_size, A.proxy, g, static_cast<int>(TC)};
-    ProxyPairs(lv, 0, PT, p.err);
+  if (!small && PT > 0) {
+    // Large groups: compact the buckets with pairs, index blocks of 2^shift
+    // pairs by their first bucket, then each pair finds its bucket with a
+    // short search in LDS and reads the bucket's packed entries: two levels
+    // of global loads per pair.
+#pragma unroll 8
+    for (uint32_t k = tid; k < TC; k += kThreads) A.nz_pos[k] = A.th_cnt[k + 1] - A.th_cnt[k] >= 2 ? 1U : 0U;
+    __syncthreads();
+    const uint32_t NB = Uniform(ScanInPlace(A.nz_pos, static_cast<int>(TC), wave_tot));
+    uint32_t* nzp = NB <= static_cast<uint32_t>(kNzLds) ? reinterpret_cast<uint32_t*>(s_region + kNzPrefOffset)
+                                                         : A.nz_pref;
+#pragma unroll 4
+    for (uint32_t k = tid; k < TC; k += kThreads) {
+      const uint32_t off = A.th_cnt[k], sz = A.th_cnt[k + 1] - off;
+      if (sz < 2) continue;
+      const uint32_t q = A.nz_pos[k];
+      A.nz_info[q] = make_uint4(off, sz, static_cast<uint32_t>(A.th_key[k]), static_cast<uint32_t>(A.th_st[k]));
+      nzp[q] = A.pair_pref[k];
+    }
+    if (tid == 0) nzp[NB] = PT;
+    __syncthreads();
+    int shift = 6;
+    while (((PT - 1) >> shift) + 1 > static_cast<uint32_t>(kBlkMax)) ++shift;
+    const uint32_t nblk = ((PT - 1) >> shift) + 1;
+    uint32_t* blk = reinterpret_cast<uint32_t*>(s_region);
+    for (uint32_t k = tid; k < NB; k += kThreads) {
+      const uint32_t pb = nzp[k], pe = nzp[k + 1];
+      for (uint32_t b = (pb + (1U << shift) - 1) >> shift; b <= ((pe - 1) >> shift); ++b) blk[b] = k;
+    }
+    __syncthreads();
+    // Four pairs per thread at a time: their loads are in flight together.
+    constexpr int kU = 4;
+    for (uint32_t base = tid; base < PT; base += kU * kThreads) {
+      uint32_t lo[kU], l[kU];
+      uint4 info[kU];
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        const uint32_t pi = min(base + u * kThreads, PT - 1);
+        const uint32_t b = pi >> shift;
+        uint32_t a = blk[b], c = b + 1 < nblk ? blk[b + 1] : NB - 1;
+        while (a < c) {  // last bucket in [a, c] whose first pair is <= pi
+          const uint32_t mid = (a + c + 1) >> 1;
+          if (nzp[mid] <= pi) a = mid; else c = mid - 1;
+        }
+        lo[u] = a;
+        l[u] = pi - nzp[a];
+      }
+#pragma unroll
+      for (int u = 0; u < kU; ++u) info[u] = A.nz_info[lo[u]];
+      unsigned long long wx[kU], wy[kU];
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        uint32_t x, y;
+        TriDecode(l[u], info[u].y, &x, &y);
+        const unsigned long long* be = reinterpret_cast<const unsigned long long*>(A.g_bucket) + info[u].x;
+        wx[u] = be[x];
+        wy[u] = be[y];
+      }
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        if (base + u * kThreads >= PT) break;
+        const int i = static_cast<int>(wx[u] >> 32), j = static_cast<int>(wy[u] >> 32);
+        const int t = static_cast<int>(info[u].z);
+        const int64_t st = static_cast<int64_t>(info[u].w);
+        const int64_t ea = static_cast<uint32_t>(wx[u]), eb = static_cast<uint32_t>(wy[u]);
+        const int64_t sa = m_size[i], sb = m_size[j];
+        const int ra = m_rep[i], rb = m_rep[j];
+        const int64_t old_cost = EncodeCost(ra, sa, t, st, ea) + EncodeCost(rb, sb, t, st, eb);
+        const int64_t new_cost = EncodeCost(ra, sa + sb, t, st, ea + eb);
+        const int64_t delta = old_cost - new_cost;
+        const int64_t sw = sa + sb;
+        const int64_t corr = BlockCost(sw, st, ea + eb, false, p.err) - BlockCost(sw, st, ea, false, p.err) -
+                             BlockCost(sw, st, eb, false, p.err);
+        const unsigned long long add = (delta > 0 ? static_cast<unsigned long long>(delta) : 0ULL) +
+                                       (static_cast<unsigned long long>(corr) << 32);
+        if (add != 0) {
+          const int a = min(i, j), c = max(i, j);
+          atomicAdd(reinterpret_cast<unsigned long long*>(&A.proxy[a * g + c]), add);
+        }
+      }
+    }
   }
+  stamp(15);
   // Phase 6: direct edges between members (sweg.cpp:1905-1924).
   for (uint32_t e = tid; e < E; e += kThreads) {
     if (A.ent_slot[e] < 0) continue;
@@ -566,16 +611,51 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
     const int wave = WaveId(), lane = LaneId();
     const int per_lane = (g + kWave - 1) / kWave;  // <= 8
     for (int i = wave; i < g; i += kWaves) {
-      int cnt = 0;
-      for (int r = 0; r < per_lane; ++r) {
+      // Keys compacted into the wave's table (two words each, partner in
+      // wj) are ranked by counting with at most two keys per lane; a key is
+      // kept when fewer than k keys beat it.
+      int16_t* wj = s_topj[WaveId()];
+      auto rank_table = [&](int n) {
+        const unsigned long long* wt =
+            reinterpret_cast<const unsigned long long*>(A.tables + WaveId() * kTableSlots * 2);
+        const bool h0 = lane < n, h1 = lane + kWave < n;
+        const unsigned long long r0 = h0 ? wt[2 * lane] : 0, k0 = h0 ? wt[2 * lane + 1] : 0;
+        const unsigned long long r1 = h1 ? wt[2 * (lane + kWave)] : 0, k1 = h1 ? wt[2 * (lane + kWave) + 1] : 0;
+        int b0 = 0, b1 = 0;
+#pragma unroll 4
+        for (int e = 0; e < n; ++e) {
+          const unsigned long long er = wt[2 * e], ep = wt[2 * e + 1];
+          b0 += (er > r0 || (er == r0 && ep > k0)) ? 1 : 0;
+          b1 += (er > r1 || (er == r1 && ep > k1)) ? 1 : 0;
+        }
+        if (h0 && b0 < topk) {
+          const int j = wj[lane];
+          const uint32_t bit = static_cast<uint32_t>(min(i, j)) * g + static_cast<uint32_t>(max(i, j));
+          atomicOr(&A.candbits[bit >> 5], 1U << (bit & 31));
+        }
+        if (h1 && b1 < topk) {
+          const int j = wj[lane + kWave];
+          const uint32_t bit = static_cast<uint32_t>(min(i, j)) * g + static_cast<uint32_t>(max(i, j));
+          atomicOr(&A.candbits[bit >> 5], 1U << (bit & 31));
+        }
+        WaveSync();
+      };
+      // The row's proxies, loaded in one batch (all in flight together).
+      long long pw[8];
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
         const int j = lane + r * kWave;
-        if (j < g && j != i && ProxyOf(A.proxy[min(i, j) * g + max(i, j)]) > 0) ++cnt;
+        pw[r] = r < per_lane && j < g && j != i ? ProxyOf(A.proxy[min(i, j) * g + max(i, j)]) : 0;
       }
+      int cnt = 0;
+#pragma unroll
+      for (int r = 0; r < 8; ++r) cnt += pw[r] > 0 ? 1 : 0;
       cnt = WaveSum32(cnt);
       if (cnt <= topk) {
-        for (int r = 0; r < per_lane; ++r) {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
           const int j = lane + r * kWave;
-          if (j < g && j != i && ProxyOf(A.proxy[min(i, j) * g + max(i, j)]) > 0) {
+          if (pw[r] > 0) {
             const uint32_t bit = static_cast<uint32_t>(min(i, j)) * g + static_cast<uint32_t>(max(i, j));
             atomicOr(&A.candbits[bit >> 5], 1U << (bit & 31));
           }
@@ -590,8 +670,8 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
         ratio[r] = 0.0;
         pv[r] = 0;
         const int j = lane + r * kWave;
-        if (r < per_lane && j < g && j != i) {
-          const long long v = ProxyOf(A.proxy[min(i, j) * g + max(i, j)]);
+        {
+          const long long v = pw[r];
           if (v > 0) {
             long long denom = m_aoc[i] + m_aoc[j];
             if (denom < 1) denom = 1;
@@ -621,29 +701,12 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
             const int pos = base + __popcll(mask & ((1ULL << lane) - 1ULL));
             wk[2 * pos] = static_cast<unsigned long long>(__double_as_longlong(ratio[r]));
             wk[2 * pos + 1] = kp[r];
+            wj[pos] = static_cast<int16_t>(lane + r * kWave);
           }
           base += __popcll(mask);
         }
         WaveSync();
-        int better[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        for (int e = 0; e < cnt; ++e) {
-          const unsigned long long er = wk[2 * e], ep = wk[2 * e + 1];
-#pragma unroll
-          for (int r = 0; r < 8; ++r) {
-            if (r >= per_lane) break;
-            const unsigned long long mr = static_cast<unsigned long long>(__double_as_longlong(ratio[r]));
-            better[r] += (er > mr || (er == mr && ep > kp[r])) ? 1 : 0;
-          }
-        }
-#pragma unroll
-        for (int r = 0; r < 8; ++r) {
-          if (((live >> r) & 1U) && better[r] < topk) {
-            const int j = lane + r * kWave;
-            const uint32_t bit = static_cast<uint32_t>(min(i, j)) * g + static_cast<uint32_t>(max(i, j));
-            atomicOr(&A.candbits[bit >> 5], 1U << (bit & 31));
-          }
-        }
-        WaveSync();
+        rank_table(cnt);
         continue;
       }
       {
@@ -668,6 +731,7 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
         wk[2 * lane + 1] = hp;
         WaveSync();
         int hrank = 0;
+        #pragma unroll 4
         for (int e = 0; e < kWave; ++e) {
           const unsigned long long er = wk[2 * e], ep = wk[2 * e + 1];
           hrank += (er > hr || (er == hr && ep > hp)) ? 1 : 0;
@@ -686,7 +750,6 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
         WaveSync();
         const unsigned long long tr = wk[2 * cap], tp = wk[2 * cap + 1];
         WaveSync();
-        uint32_t keep = 0;
         int base = 0;
 #pragma unroll
         for (int r = 0; r < 8; ++r) {
@@ -695,36 +758,18 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
           const bool have = ((live >> r) & 1U) && (kr > tr || (kr == tr && kp[r] >= tp));
           const unsigned long long mask = __ballot(have);
           if (have) {
-            keep |= 1U << r;
             const int pos = base + __popcll(mask & ((1ULL << lane) - 1ULL));
             if (pos < cap) {
               wk[2 * pos] = kr;
               wk[2 * pos + 1] = kp[r];
+              wj[pos] = static_cast<int16_t>(lane + r * kWave);
             }
           }
           base += __popcll(mask);
         }
         WaveSync();
         if (base <= cap) {
-          int better[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-          for (int e = 0; e < base; ++e) {
-            const unsigned long long er = wk[2 * e], ep = wk[2 * e + 1];
-#pragma unroll
-            for (int r = 0; r < 8; ++r) {
-              if (r >= per_lane) break;
-              const unsigned long long mr = static_cast<unsigned long long>(__double_as_longlong(ratio[r]));
-              better[r] += (er > mr || (er == mr && ep > kp[r])) ? 1 : 0;
-            }
-          }
-#pragma unroll
-          for (int r = 0; r < 8; ++r) {
-            if (((keep >> r) & 1U) && better[r] < topk) {
-              const int j = lane + r * kWave;
-              const uint32_t bit = static_cast<uint32_t>(min(i, j)) * g + static_cast<uint32_t>(max(i, j));
-              atomicOr(&A.candbits[bit >> 5], 1U << (bit & 31));
-            }
-          }
-          WaveSync();
+          rank_table(base);
           continue;
         }
       }

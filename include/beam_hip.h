@@ -153,7 +153,8 @@ int beam_hip_kernel_profile(beam_hip_engine* engine, double* out);
  * order, at most 4 G + 1024 records), 16 values: g, staged entries, bucket
  * pairs, candidates, survivors, then wall-clock stamps (100 MHz) at entry and
  * after members, staging, proxy accumulation, top-k, candidate listing,
- * scoring, selection and queueing, then the group index and 0. */
+ * scoring, selection and queueing, then the group index and a stamp after
+ * the bucket-pair phase. */
 int beam_hip_trace(beam_hip_engine* engine, int32_t iter);
 /* Copies up to cap_groups trace records; returns the record count. */
 int64_t beam_hip_trace_fetch(beam_hip_engine* engine, int64_t* out, int64_t cap_groups);
