@@ -22,6 +22,8 @@ thread_local std::string g_error;
 
 constexpr int kMaxIterations = 4096;
 constexpr int kCommitGrid = 512;
+constexpr int kSplitCap = 4096;       // split groups per round
+constexpr int kSplitItems = 1 << 18;  // pair chunks (and row chunks) per round
 // Learned ranker constants.
 constexpr int kRankerMinKeep = 8;                 // groups with at most this many candidates keep all
 constexpr int kAuditGroups = 1024;                // groups audited per pruned iteration
@@ -255,6 +257,7 @@ __global__ void InitRound(RoundState* rs, int end, int32_t* naff, unsigned long 
   rs->j = 0;
   rs->ntodo = 0;
   rs->steal = 0;
+  rs->nsplit = rs->npair_items = rs->nrow_items = rs->steal_pairs = rs->steal_rows = 0;
   *naff = 0;
   *group_top = 0;
 }
@@ -483,6 +486,9 @@ struct Engine {
   int32_t* gmark = nullptr;       // rounds: epoch of the queue holding each group
   int32_t* todo[2] = {nullptr, nullptr};
   RoundState* rs = nullptr;       // [2], alternating between rounds
+  SplitRec* split = nullptr;      // [kSplitCap] split groups of a round
+  int2* pair_items = nullptr;     // [kSplitItems]
+  int2* row_items = nullptr;      // [kSplitItems]
   unsigned long long* round_ctr = nullptr;  // [2] productive rounds, groups scored
   int horizon_blocks = 64;
   int score_grid = 256;
@@ -509,6 +515,7 @@ struct Engine {
 
   // Host-side bookkeeping
   int64_t divide_max_group = 0;
+  int64_t last_max_group = 0;     // largest group of the current iteration
   int64_t divide_fallback_splits = 0;
   std::vector<int64_t> round_rows;  // 10 per iteration
   std::vector<Counters> iter_ctr;
@@ -569,7 +576,7 @@ Engine::~Engine() {
                   open_rep, mh,       recs[0],  recs[1],   flag,     excl,       rid,       run_start, run_open,
                   run_seg,  elem_open, new_idx, gflag,     gstart,   glen,       d_total,   d_maxlen, sel,
                   sel_count, mark_dest, dest,   mark_keep, mark_row, aff,        naff,      scratch_a, scratch_u,
-                  slot_stats, touch,  commit_out, node_group, gmark,     todo[0],   todo[1], rs, round_ctr, samples, sample_target, sample_key,
+                  slot_stats, touch,  commit_out, node_group, gmark,     todo[0],   todo[1], rs, split, pair_items, row_items, round_ctr, samples, sample_target, sample_key,
                   sample_count, audit_out, audit_list, audit_rs,
                   tops,     ctr,      err,      enc,        trace_buf};
   for (void* p : ptrs)
@@ -654,6 +661,9 @@ void Engine::Create(const int64_t* h_row_ptr, const int32_t* h_col, int n_, long
   todo[0] = DevAlloc<int32_t>(n);
   todo[1] = DevAlloc<int32_t>(n);
   rs = DevAlloc<RoundState>(2);
+  split = DevAlloc<SplitRec>(kSplitCap);
+  pair_items = DevAlloc<int2>(kSplitItems);
+  row_items = DevAlloc<int2>(kSplitItems);
   round_ctr = DevAlloc<unsigned long long>(2);
   {
     int cus = 0, per_cu = 0;
@@ -861,6 +871,7 @@ int Engine::Divide(int iter) {
   HIP_CHECK(hipMemcpyAsync(&maxlen, d_maxlen, sizeof(maxlen), hipMemcpyDeviceToHost, stream));
   Sync();
   divide_max_group = std::max<int64_t>(divide_max_group, static_cast<int64_t>(maxlen));
+  last_max_group = static_cast<int64_t>(maxlen);
   last_groups = G;
   return G;
 }
@@ -942,6 +953,12 @@ void Engine::LaunchRound(int G, int nblocks, int block_size, double theta, Count
   mp.trace = (trace_buf != nullptr && tracing_now) ? trace_buf : nullptr;
   mp.round = cur;
   mp.todo = todo[epoch & 1];
+  const bool split_now = last_max_group >= kSplitG;
+  mp.split = split_now ? split : nullptr;
+  mp.pair_items = pair_items;
+  mp.row_items = row_items;
+  mp.split_cap = kSplitCap;
+  mp.item_cap = kSplitItems;
   SetRankerParams(mp);
   if (profile) {
     if (ev_used + 2 > ev_pool.size()) {
@@ -953,10 +970,21 @@ void Engine::LaunchRound(int G, int nblocks, int block_size, double theta, Count
     }
     HIP_CHECK(hipEventRecord(ev_pool[ev_used++], stream));
   }
+  // In iterations with groups of at least kSplitG members, the group kernel
+  // stages those and the pair and row kernels finish them (both exit at
+  // once in rounds without any).
   if (ranker_mode_now != kRankOff) {
     MergeGroupsRankerKernel<<<score_grid, 512, 0, stream>>>(mp);
+    if (split_now) {
+      SplitPairsKernel<<<score_grid, 512, 0, stream>>>(mp);
+      SplitRowsRankerKernel<<<score_grid, 512, 0, stream>>>(mp);
+    }
   } else {
     MergeGroupsKernel<<<score_grid, 512, 0, stream>>>(mp);
+    if (split_now) {
+      SplitPairsKernel<<<score_grid, 512, 0, stream>>>(mp);
+      SplitRowsKernel<<<score_grid, 512, 0, stream>>>(mp);
+    }
   }
   if (profile) HIP_CHECK(hipEventRecord(ev_pool[ev_used++], stream));
 
@@ -1094,7 +1122,7 @@ void Engine::RankerSetupIteration(int iter, int G, double theta) {
   for (int k = 0; k < count; ++k) list[static_cast<size_t>(k)] = k * stride;
   HIP_CHECK(hipMemcpyAsync(audit_list, list.data(), sizeof(int32_t) * count, hipMemcpyHostToDevice, stream));
   HIP_CHECK(hipMemsetAsync(audit_out, 0, sizeof(long long) * 3 * count, stream));
-  const RoundState init{0, 0, 0, count, 0, {0, 0, 0}};
+  const RoundState init{0, 0, 0, count, 0, 0, 0, 0, 0, 0, {0, 0, 0}};
   HIP_CHECK(hipMemcpyAsync(audit_rs, &init, sizeof(init), hipMemcpyHostToDevice, stream));
   HIP_CHECK(hipMemsetAsync(&tops[0], 0, sizeof(unsigned long long), stream));
   MergeParams mp{};

@@ -70,7 +70,34 @@ struct RoundState {
   int j;       // blocks this round commits
   int ntodo;   // groups queued for scoring in this round
   int steal;   // scoring work counter
+  // Groups of at least kSplitG members are split over workgroups: the group
+  // kernel stages them (front), the pair kernel expands their bucket pairs
+  // in chunks, the top-k kernel ranks their rows in chunks and the
+  // workgroup that finishes a group's last row chunk scores and selects it.
+  int nsplit;        // split groups (records) of this round
+  int npair_items;   // bucket-pair chunks queued
+  int nrow_items;    // row chunks queued
+  int steal_pairs;   // pair kernel work counter
+  int steal_rows;    // top-k kernel work counter
   int pad[3];
+};
+
+constexpr int kSplitG = 128;
+
+// One split group of a round (written by its front, read by the kernels
+// after it).
+struct SplitRec {
+  int gidx;
+  int item;                 // the group's position in the round queue
+  int E;                    // staged entries
+  uint32_t PT;              // bucket pairs
+  uint32_t NB;              // buckets with pairs
+  int shift;                // the block index covers 2^shift pairs per block
+  uint32_t nblk;
+  int row_chunks;
+  unsigned long long base;  // offset of the group's arrays in the scratch pool
+  int rows_done;            // row chunks finished
+  int ok;                   // 1 once the front has finished
 };
 
 struct MergeParams {
@@ -105,6 +132,12 @@ struct MergeParams {
   // and write sel / sel_count / slot_stats per group instead of per slot.
   RoundState* round;
   const int32_t* todo;
+  // Split groups (round mode; nullptr: every group on one workgroup).
+  SplitRec* split;
+  int2* pair_items;         // (record, chunk)
+  int2* row_items;          // (record, chunk); record < 0: unused
+  int split_cap;            // records
+  int item_cap;             // entries of each item list
   // Learned ranker (kRankOff: plain exact scoring of every candidate).
   int ranker_mode;
   RankerView ranker;
@@ -196,6 +229,9 @@ struct CommitParams {
 
 __global__ void MergeGroupsKernel(MergeParams p);        // plain exact scoring
 __global__ void MergeGroupsRankerKernel(MergeParams p);  // with the learned ranker's modes
+__global__ void SplitPairsKernel(MergeParams p);         // bucket-pair chunks of split groups
+__global__ void SplitRowsKernel(MergeParams p);          // top-k row chunks, then the group's scoring
+__global__ void SplitRowsRankerKernel(MergeParams p);
 __global__ void CommitKernel(CommitParams p);
 __global__ void TouchRound(RoundParams p);
 __global__ void FindCommitBlocks(RoundParams p);

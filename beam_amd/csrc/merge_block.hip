@@ -50,6 +50,8 @@ static_assert(kMaxGroup * 16 * 8 <= kRegionBytes, "large groups' survivor keys f
 // (first non-empty bucket of every block of 2^shift pairs, at most kBlkMax
 // blocks) and, when they fit, the compacted buckets' pair prefix.
 constexpr int kBlkMax = 4096;
+constexpr int kBlkMaxBytes = (kBlkMax + 1) * 4;
+static_assert((kMaxGroup * kMaxGroup / 32 + 1) * 4 <= kTableOffset, "large groups' kept-pair bits fit below the join tables");
 constexpr int kNzPrefOffset = ((kBlkMax + 1) * 4 + 15) & ~15;
 constexpr int kNzLds = (kRegionBytes - kNzPrefOffset) / 4 - 1;
 static_assert(kNzLds >= 8192, "the LDS region holds a useful compacted-bucket prefix");
@@ -83,7 +85,16 @@ struct GroupArrays {
   uint32_t* nz_pos;
   uint4* nz_info;
   uint32_t* nz_pref;
+  // Large groups: the bucket-pair block index and the members' approximate
+  // old costs, reps and sizes, in the scratch (used across kernels when the
+  // group is split).
+  uint32_t* blk;
+  long long* maoc;
+  int32_t* mrep;
+  int32_t* msize;
 };
+
+
 
 // Exclusive scan in place over n entries of `data` (generic pointer), with
 // data[n] receiving the total.  All threads of the workgroup participate.
@@ -141,6 +152,118 @@ __device__ __forceinline__ long long ProxyOf(long long w) {
   return static_cast<long long>(static_cast<unsigned long long>(w) & 0xffffffffULL);
 }
 __device__ __forceinline__ long long CorrOf(long long w) { return w >> 32; }
+
+// Scratch bytes of a large group and the layout of its arrays.
+__device__ __forceinline__ unsigned long long LargeBytes(uint32_t E, uint32_t TC, int g, uint32_t cmax) {
+  const unsigned long long e64 = E, tc64 = TC, g64 = static_cast<unsigned long long>(g), c64 = cmax;
+  return e64 * 4 * 8 + tc64 * 4 + (tc64 + 1) * 8 + tc64 * 4 + (tc64 + 1) * 4 + 16 + tc64 * 16 + (tc64 + 1) * 4 + 16 +
+         g64 * g64 * 8 + (g64 * g64 / 32 + 1) * 4 + 16 + c64 * (8 + 8 + 2 + 2) + (c64 + 1) * 4 + g64 * 64 + e64 * 4 +
+         64 + (kBlkMaxBytes + 16) + g64 * 16 + 16;
+}
+
+__device__ __forceinline__ void LargeLayout(unsigned char* r, uint32_t E, uint32_t TC, int g, uint32_t cmax,
+                                            GroupArrays& A) {
+  auto align8 = [](unsigned char* x) {
+    return reinterpret_cast<unsigned char*>((reinterpret_cast<uintptr_t>(x) + 7) & ~uintptr_t(7));
+  };
+  A.ent_t = reinterpret_cast<int32_t*>(r); r += E * 4ULL;
+  A.ent_e = reinterpret_cast<int32_t*>(r); r += E * 4ULL;
+  A.ent_st = reinterpret_cast<int32_t*>(r); r += E * 4ULL;
+  A.ent_slot = reinterpret_cast<int32_t*>(r); r += E * 4ULL;
+  A.g_ent_i = reinterpret_cast<int32_t*>(r); r += E * 4ULL;
+  A.g_ent_rank = reinterpret_cast<int32_t*>(r); r += E * 4ULL;
+  A.g_bucket = reinterpret_cast<int32_t*>(r); r += E * 8ULL;  // 8-aligned: 24 E bytes precede it
+  A.th_key = reinterpret_cast<int32_t*>(r); r += TC * 4ULL;
+  A.th_cnt = reinterpret_cast<uint32_t*>(r); r += (TC + 1) * 4ULL;
+  A.pair_pref = reinterpret_cast<uint32_t*>(r); r += (TC + 1) * 4ULL;
+  A.th_st = reinterpret_cast<int32_t*>(r); r += TC * 4ULL;
+  A.nz_pos = reinterpret_cast<uint32_t*>(r); r += (TC + 1) * 4ULL;
+  r = reinterpret_cast<unsigned char*>((reinterpret_cast<uintptr_t>(r) + 15) & ~uintptr_t(15));
+  A.nz_info = reinterpret_cast<uint4*>(r); r += TC * 16ULL;
+  A.nz_pref = reinterpret_cast<uint32_t*>(r); r += (TC + 1) * 4ULL;
+  r = align8(r);
+  A.proxy = reinterpret_cast<long long*>(r); r += static_cast<unsigned long long>(g) * g * 8;
+  A.candbits = reinterpret_cast<uint32_t*>(r); r += (static_cast<unsigned long long>(g) * g / 32 + 1) * 4;
+  r = align8(r);
+  A.cand_gain = reinterpret_cast<long long*>(r); r += cmax * 8ULL;
+  A.cand_before = reinterpret_cast<long long*>(r); r += cmax * 8ULL;
+  A.pass = reinterpret_cast<uint32_t*>(r); r += (cmax + 1) * 4ULL;
+  A.cand_a = reinterpret_cast<uint16_t*>(r); r += cmax * 2ULL;
+  A.cand_b = reinterpret_cast<uint16_t*>(r); r += cmax * 2ULL;
+  r = align8(r);
+  A.sig = reinterpret_cast<uint32_t*>(r); r += static_cast<unsigned long long>(g) * 64;
+  A.spec = reinterpret_cast<uint32_t*>(r); r += E * 4ULL + 64;
+  r = align8(r);
+  A.blk = reinterpret_cast<uint32_t*>(r); r += kBlkMaxBytes + 16;
+  r = align8(r);
+  A.maoc = reinterpret_cast<long long*>(r); r += static_cast<unsigned long long>(g) * 8;
+  A.mrep = reinterpret_cast<int32_t*>(r); r += static_cast<unsigned long long>(g) * 4;
+  A.msize = reinterpret_cast<int32_t*>(r);
+}
+
+// Bucket pairs [p0, p1) of a large group (sweg.cpp:1863-1903): for two
+// members sharing target t, delta = EC(a,t,ea) + EC(b,t,eb) - EC(a+b,t,ea+eb)
+// is accumulated when positive, and t's term of the pair's exact-gain
+// common-neighbour correction goes into the high word (one atomic add for
+// both; order-free).  A pair finds its bucket with the block index (blk, the
+// first non-empty bucket of each block of 2^shift pairs) and a short search
+// of the compacted buckets' pair prefix (nzp); four pairs per thread are in
+// flight together.  All threads of the workgroup.
+__device__ __forceinline__ void BucketPairs(const GroupArrays& A, const uint32_t* blk, const uint32_t* nzp,
+                                            const int32_t* msize, const int32_t* mrep, int g, uint32_t PT,
+                                            uint32_t NB, int shift, uint32_t nblk, uint32_t p0, uint32_t p1,
+                                            int* err) {
+  constexpr int kU = 4;
+  for (uint32_t base = p0 + threadIdx.x; base < p1; base += kU * kThreads) {
+    uint32_t lo[kU], l[kU];
+    uint4 info[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const uint32_t pi = min(base + u * kThreads, p1 - 1);
+      const uint32_t b = pi >> shift;
+      uint32_t a = blk[b], c = b + 1 < nblk ? blk[b + 1] : NB - 1;
+      while (a < c) {  // last bucket in [a, c] whose first pair is <= pi
+        const uint32_t mid = (a + c + 1) >> 1;
+        if (nzp[mid] <= pi) a = mid; else c = mid - 1;
+      }
+      lo[u] = a;
+      l[u] = pi - nzp[a];
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u) info[u] = A.nz_info[lo[u]];
+    unsigned long long wx[kU], wy[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      uint32_t x, y;
+      TriDecode(l[u], info[u].y, &x, &y);
+      const unsigned long long* be = reinterpret_cast<const unsigned long long*>(A.g_bucket) + info[u].x;
+      wx[u] = be[x];
+      wy[u] = be[y];
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      if (base + u * kThreads >= p1) break;
+      const int i = static_cast<int>(wx[u] >> 32), j = static_cast<int>(wy[u] >> 32);
+      const int t = static_cast<int>(info[u].z);
+      const int64_t st = static_cast<int64_t>(info[u].w);
+      const int64_t ea = static_cast<uint32_t>(wx[u]), eb = static_cast<uint32_t>(wy[u]);
+      const int64_t sa = msize[i], sb = msize[j];
+      const int ra = mrep[i], rb = mrep[j];
+      const int64_t old_cost = EncodeCost(ra, sa, t, st, ea) + EncodeCost(rb, sb, t, st, eb);
+      const int64_t new_cost = EncodeCost(ra, sa + sb, t, st, ea + eb);
+      const int64_t delta = old_cost - new_cost;
+      const int64_t sw = sa + sb;
+      const int64_t corr = BlockCost(sw, st, ea + eb, false, err) - BlockCost(sw, st, ea, false, err) -
+                           BlockCost(sw, st, eb, false, err);
+      const unsigned long long add = (delta > 0 ? static_cast<unsigned long long>(delta) : 0ULL) +
+                                     (static_cast<unsigned long long>(corr) << 32);
+      if (add != 0) {
+        const int a = min(i, j), c = max(i, j);
+        atomicAdd(reinterpret_cast<unsigned long long*>(&A.proxy[a * g + c]), add);
+      }
+    }
+  }
+}
 
 // What the ranker's features read from a staged group.
 struct FeatureSource {
@@ -216,19 +339,231 @@ __device__ __forceinline__ void RankerPredict(const FeatureSource& fsrc, const R
   }
 }
 
+// Top-k of member i's row (sweg.cpp:1940-1977) for a row on another
+// workgroup than the group's: pw holds the row's proxies (lane + 64 r),
+// tables the wave's join table, wj its partner slots.
+__device__ __forceinline__ void TopkRowRank(int i, int g, int topk, int per_lane, const long long* pw,
+                                            const long long* aoc, const int32_t* rep, uint32_t* candbits,
+                                            int32_t* tables, int16_t* wj) {
+  const int lane = LaneId();
+  // Keys compacted into the wave's table (two words each, partner in
+  // wj) are ranked by counting with at most two keys per lane; a key is
+  // kept when fewer than k keys beat it.
+  auto rank_table = [&](int n) {
+    const unsigned long long* wt =
+        reinterpret_cast<const unsigned long long*>(tables + WaveId() * kTableSlots * 2);
+    const bool h0 = lane < n, h1 = lane + kWave < n;
+    const unsigned long long r0 = h0 ? wt[2 * lane] : 0, k0 = h0 ? wt[2 * lane + 1] : 0;
+    const unsigned long long r1 = h1 ? wt[2 * (lane + kWave)] : 0, k1 = h1 ? wt[2 * (lane + kWave) + 1] : 0;
+    int b0 = 0, b1 = 0;
+#pragma unroll 4
+    for (int e = 0; e < n; ++e) {
+      const unsigned long long er = wt[2 * e], ep = wt[2 * e + 1];
+      b0 += (er > r0 || (er == r0 && ep > k0)) ? 1 : 0;
+      b1 += (er > r1 || (er == r1 && ep > k1)) ? 1 : 0;
+    }
+    if (h0 && b0 < topk) {
+      const int j = wj[lane];
+      const uint32_t bit = static_cast<uint32_t>(min(i, j)) * g + static_cast<uint32_t>(max(i, j));
+      atomicOr(&candbits[bit >> 5], 1U << (bit & 31));
+    }
+    if (h1 && b1 < topk) {
+      const int j = wj[lane + kWave];
+      const uint32_t bit = static_cast<uint32_t>(min(i, j)) * g + static_cast<uint32_t>(max(i, j));
+      atomicOr(&candbits[bit >> 5], 1U << (bit & 31));
+    }
+    WaveSync();
+  };
+  int cnt = 0;
+#pragma unroll
+  for (int r = 0; r < 8; ++r) cnt += pw[r] > 0 ? 1 : 0;
+  cnt = WaveSum32(cnt);
+  if (cnt <= topk) {
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const int j = lane + r * kWave;
+      if (pw[r] > 0) {
+        const uint32_t bit = static_cast<uint32_t>(min(i, j)) * g + static_cast<uint32_t>(max(i, j));
+        atomicOr(&candbits[bit >> 5], 1U << (bit & 31));
+      }
+    }
+    return;
+  }
+  double ratio[8];
+  long long pv[8];
+  uint32_t live = 0;
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    ratio[r] = 0.0;
+    pv[r] = 0;
+    const int j = lane + r * kWave;
+    {
+      const long long v = pw[r];
+      if (v > 0) {
+        long long denom = aoc[i] + aoc[j];
+        if (denom < 1) denom = 1;
+        ratio[r] = static_cast<double>(v) / static_cast<double>(denom);
+        pv[r] = v;
+        live |= 1U << r;
+      }
+    }
+  }
+  if (cnt <= kRankMax) {
+    // Rank by counting: key = (ratio bits, proxy << 32 | ~rep_j), larger
+    // is better (positive doubles order as their bit patterns; the proxy
+    // is below 2^32, see ProxyOf).  The row's
+    // keys are compacted into the wave's table; a candidate is kept when
+    // fewer than k keys beat it.
+    unsigned long long* wk = reinterpret_cast<unsigned long long*>(tables + WaveId() * kTableSlots * 2);
+    unsigned long long kp[8];
+    int base = 0;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      kp[r] = (static_cast<unsigned long long>(pv[r]) << 32) |
+              static_cast<unsigned long long>(0xffffffffU - static_cast<uint32_t>(rep[min(g - 1, lane + r * kWave)]));
+      if (r >= per_lane) continue;
+      const bool have = (live >> r) & 1U;
+      const unsigned long long mask = __ballot(have);
+      if (have) {
+        const int pos = base + __popcll(mask & ((1ULL << lane) - 1ULL));
+        wk[2 * pos] = static_cast<unsigned long long>(__double_as_longlong(ratio[r]));
+        wk[2 * pos + 1] = kp[r];
+        wj[pos] = static_cast<int16_t>(lane + r * kWave);
+      }
+      base += __popcll(mask);
+    }
+    WaveSync();
+    rank_table(cnt);
+    return;
+  }
+  {
+    // More candidates than the table holds: only keys at least the k-th
+    // best of the lanes' best keys (k distinct candidates) can be among
+    // the row's top k, so only those are ranked.  With more than 128
+    // candidates on at most 8 per lane, at least 17 lanes hold one.
+    unsigned long long* wk = reinterpret_cast<unsigned long long*>(tables + WaveId() * kTableSlots * 2);
+    unsigned long long kp[8], hr = 0, hp = 0;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      kp[r] = (static_cast<unsigned long long>(pv[r]) << 32) |
+              static_cast<unsigned long long>(0xffffffffU - static_cast<uint32_t>(rep[min(g - 1, lane + r * kWave)]));
+      if (!((live >> r) & 1U)) continue;
+      const unsigned long long kr = static_cast<unsigned long long>(__double_as_longlong(ratio[r]));
+      if (kr > hr || (kr == hr && kp[r] > hp)) {
+        hr = kr;
+        hp = kp[r];
+      }
+    }
+    wk[2 * lane] = hr;
+    wk[2 * lane + 1] = hp;
+    WaveSync();
+    int hrank = 0;
+    #pragma unroll 4
+    for (int e = 0; e < kWave; ++e) {
+      const unsigned long long er = wk[2 * e], ep = wk[2 * e + 1];
+      hrank += (er > hr || (er == hr && ep > hp)) ? 1 : 0;
+    }
+    WaveSync();
+    const int cap = kRankMax - 1;  // the last key slot carries the threshold
+    if (lane == 0) {
+      wk[2 * cap] = 0;  // no threshold (fewer valid lanes than k): rank all, below
+      wk[2 * cap + 1] = 0;
+    }
+    WaveSync();
+    if (hr != 0 && hrank == topk - 1) {
+      wk[2 * cap] = hr;
+      wk[2 * cap + 1] = hp;
+    }
+    WaveSync();
+    const unsigned long long tr = wk[2 * cap], tp = wk[2 * cap + 1];
+    WaveSync();
+    int base = 0;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      if (r >= per_lane) continue;
+      const unsigned long long kr = static_cast<unsigned long long>(__double_as_longlong(ratio[r]));
+      const bool have = ((live >> r) & 1U) && (kr > tr || (kr == tr && kp[r] >= tp));
+      const unsigned long long mask = __ballot(have);
+      if (have) {
+        const int pos = base + __popcll(mask & ((1ULL << lane) - 1ULL));
+        if (pos < cap) {
+          wk[2 * pos] = kr;
+          wk[2 * pos + 1] = kp[r];
+          wj[pos] = static_cast<int16_t>(lane + r * kWave);
+        }
+      }
+      base += __popcll(mask);
+    }
+    WaveSync();
+    if (base <= cap) {
+      rank_table(base);
+      return;
+    }
+  }
+  for (int round = 0; round < topk; ++round) {
+    double br = 0.0;
+    long long bp = 0;
+    int brep = 0x7fffffff, bslot = -1;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      if (!((live >> r) & 1U)) continue;
+      const int rj = rep[lane + r * kWave];
+      if (bslot < 0 || ratio[r] > br || (ratio[r] == br && (pv[r] > bp || (pv[r] == bp && rj < brep)))) {
+        br = ratio[r];
+        bp = pv[r];
+        brep = rj;
+        bslot = r;
+      }
+    }
+    int valid = bslot >= 0 ? 1 : 0;
+    int wlane = lane;
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+      const double orat = __shfl_xor(br, off, kWave);
+      const long long opv = __shfl_xor(bp, off, kWave);
+      const int orep = __shfl_xor(brep, off, kWave);
+      const int ovalid = __shfl_xor(valid, off, kWave);
+      const int olane = __shfl_xor(wlane, off, kWave);
+      const bool take = ovalid && (!valid || orat > br || (orat == br && (opv > bp || (opv == bp && orep < brep))));
+      if (take) {
+        br = orat;
+        bp = opv;
+        brep = orep;
+        valid = ovalid;
+        wlane = olane;
+      }
+    }
+    if (!valid) break;
+    if (lane == wlane) {
+      live &= ~(1U << bslot);
+      const int j = lane + bslot * kWave;
+      const uint32_t bit = static_cast<uint32_t>(min(i, j)) * g + static_cast<uint32_t>(max(i, j));
+      atomicOr(&candbits[bit >> 5], 1U << (bit & 31));
+    }
+  }
+}
+
+enum : int {
+  kModeFull = 0,   // every phase on this workgroup
+  kModeFront = 1,  // split group: staging, buckets and direct edges; pairs and rows go to other kernels
+  kModeBack = 2,   // split group: candidate listing, exact scoring and selection
+};
+constexpr uint32_t kPairChunk = 4096;  // bucket pairs per pair-kernel item
+
 // Candidate generation, exact scoring and selection for group gidx; results
 // go to sel_out (pairs), sel_count[slot], slot_stats[slot].  Every early exit
-// is taken by the whole workgroup.
+// is taken by the whole workgroup.  In round mode with split records, a group
+// of at least kSplitG members becomes a front (its record is rec).
 template <bool kRanker>
 __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int slot, int32_t* sel_out, int item,
-                                           unsigned char* s_region) {
+                                           unsigned char* s_region, int mode, SplitRec* rec) {
   long long* trace = nullptr;
   const long long t_start = static_cast<long long>(wall_clock64());
   auto stamp = [&](int k) {
     if (trace != nullptr && threadIdx.x == 0) trace[k] = static_cast<long long>(wall_clock64());
   };
   const int tid = static_cast<int>(threadIdx.x);
-  if (tid == 0) {
+  if (tid == 0 && mode != kModeBack) {
     p.sel_count[slot] = 0;
     p.slot_stats[static_cast<long long>(slot) * kSlotStats + kStatGroup] = 0;
   }
@@ -237,6 +572,30 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
   if (g > kMaxGroup) {
     if (tid == 0) atomicOr(p.err, kErrSelection);
     return;
+  }
+  __shared__ int s_mode, s_rslot, s_pb;
+  if (mode == kModeFull && p.split != nullptr && g >= kSplitG) {
+    // Reserve a record and the row chunks; without room the group stays on
+    // this workgroup (reserved row entries are then marked unused).
+    if (tid == 0) {
+      const int rc = (g + kWaves - 1) / kWaves;
+      const int k = atomicAdd(&p.round->nsplit, 1);
+      const int rb = atomicAdd(&p.round->nrow_items, rc);
+      const bool ok = k < p.split_cap && rb + rc <= p.item_cap;
+      for (int c = 0; c < rc && rb + c < p.item_cap; ++c) p.row_items[rb + c] = make_int2(ok ? k : -1, c);
+      if (ok) {
+        SplitRec* r = p.split + k;
+        r->gidx = gidx;
+        r->item = item;
+        r->row_chunks = rc;
+        r->ok = 0;
+      }
+      s_mode = ok ? kModeFront : kModeFull;
+      s_rslot = ok ? k : -1;
+    }
+    __syncthreads();
+    mode = s_mode;
+    if (mode == kModeFront) rec = p.split + s_rslot;
   }
   if (p.trace != nullptr && tid == 0) {
     // Record 0 holds the record count and capacity; one record per scoring.
@@ -327,496 +686,479 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
     A.pass = reinterpret_cast<uint32_t*>(c); c += (kSmallC + 1) * 4;
     A.cand_a = reinterpret_cast<uint16_t*>(c); c += kSmallC * 2;
     A.cand_b = reinterpret_cast<uint16_t*>(c);
-    A.g_ent_i = A.g_ent_rank = A.g_bucket = A.th_st = nullptr;
-    A.nz_pos = A.nz_pref = nullptr;
+    A.g_ent_i = A.g_ent_rank = A.g_bucket = A.th_st = A.mrep = A.msize = nullptr;
+    A.nz_pos = A.nz_pref = A.blk = nullptr;
     A.nz_info = nullptr;
+    A.maoc = nullptr;
     A.sig = s_sig;
     A.spec = nullptr;
   } else {
     if (tid == 0) {
-      const unsigned long long e64 = E, tc64 = TC, g64 = static_cast<unsigned long long>(g), c64 = cmax;
-      const unsigned long long bytes = e64 * 4 * 8 + tc64 * 4 + (tc64 + 1) * 8 + tc64 * 4 + (tc64 + 1) * 4 + 16 +
-                                       tc64 * 16 + (tc64 + 1) * 4 + 16 + g64 * g64 * 8 +
-                                       (g64 * g64 / 32 + 1) * 4 + 16 + c64 * (8 + 8 + 2 + 2) +
-                                       (c64 + 1) * 4 + g64 * 64 + e64 * 4 + 64;
-      const unsigned long long base = atomicAdd(p.scratch_top, (bytes + 255) & ~255ULL);
-      s_base = base;
-      s_flag = base + bytes > p.scratch_cap ? 1 : 0;
-      if (s_flag) atomicOr(p.err, kErrScratch);
+      if (mode == kModeBack) {
+        s_base = rec->base;
+        s_flag = 0;
+      } else {
+        const unsigned long long bytes = LargeBytes(E, TC, g, cmax);
+        const unsigned long long base = atomicAdd(p.scratch_top, (bytes + 255) & ~255ULL);
+        s_base = base;
+        s_flag = base + bytes > p.scratch_cap ? 1 : 0;
+        if (s_flag) atomicOr(p.err, kErrScratch);
+      }
     }
     __syncthreads();
-    if (s_flag) return;
-    unsigned char* r = p.scratch + Uniform(s_base);
-    auto align8 = [](unsigned char* x) {
-      return reinterpret_cast<unsigned char*>((reinterpret_cast<uintptr_t>(x) + 7) & ~uintptr_t(7));
-    };
-    A.ent_t = reinterpret_cast<int32_t*>(r); r += E * 4ULL;
-    A.ent_e = reinterpret_cast<int32_t*>(r); r += E * 4ULL;
-    A.ent_st = reinterpret_cast<int32_t*>(r); r += E * 4ULL;
-    A.ent_slot = reinterpret_cast<int32_t*>(r); r += E * 4ULL;
-    A.g_ent_i = reinterpret_cast<int32_t*>(r); r += E * 4ULL;
-    A.g_ent_rank = reinterpret_cast<int32_t*>(r); r += E * 4ULL;
-    A.g_bucket = reinterpret_cast<int32_t*>(r); r += E * 8ULL;  // 8-aligned: 24 E bytes precede it
-    A.th_key = reinterpret_cast<int32_t*>(r); r += TC * 4ULL;
-    A.th_cnt = reinterpret_cast<uint32_t*>(r); r += (TC + 1) * 4ULL;
-    A.pair_pref = reinterpret_cast<uint32_t*>(r); r += (TC + 1) * 4ULL;
-    A.th_st = reinterpret_cast<int32_t*>(r); r += TC * 4ULL;
-    A.nz_pos = reinterpret_cast<uint32_t*>(r); r += (TC + 1) * 4ULL;
-    r = reinterpret_cast<unsigned char*>((reinterpret_cast<uintptr_t>(r) + 15) & ~uintptr_t(15));
-    A.nz_info = reinterpret_cast<uint4*>(r); r += TC * 16ULL;
-    A.nz_pref = reinterpret_cast<uint32_t*>(r); r += (TC + 1) * 4ULL;
-    r = align8(r);
-    A.proxy = reinterpret_cast<long long*>(r); r += static_cast<unsigned long long>(g) * g * 8;
-    A.candbits = reinterpret_cast<uint32_t*>(r); r += (static_cast<unsigned long long>(g) * g / 32 + 1) * 4;
-    r = align8(r);
-    A.cand_gain = reinterpret_cast<long long*>(r); r += cmax * 8ULL;
-    A.cand_before = reinterpret_cast<long long*>(r); r += cmax * 8ULL;
-    A.pass = reinterpret_cast<uint32_t*>(r); r += (cmax + 1) * 4ULL;
-    A.cand_a = reinterpret_cast<uint16_t*>(r); r += cmax * 2ULL;
-    A.cand_b = reinterpret_cast<uint16_t*>(r); r += cmax * 2ULL;
-    r = align8(r);
-    A.sig = reinterpret_cast<uint32_t*>(r); r += static_cast<unsigned long long>(g) * 64;
-    A.spec = reinterpret_cast<uint32_t*>(r);
+    if (s_flag) return;  // a front's record stays not ok: the other kernels skip it
+    LargeLayout(p.scratch + Uniform(s_base), E, TC, g, cmax, A);
   }
   int16_t* s_ent_rank = reinterpret_cast<int16_t*>(s_region + kSmallE * 4);
   int16_t* s_bucket = s_ent_rank + kSmallE;
 
-  // Clear the target table, the proxy matrix and the candidate bits.
-  for (uint32_t k = tid; k < TC; k += kThreads) {
-    A.th_key[k] = -1;
-    A.th_cnt[k] = 0;
-  }
-  const uint32_t gg = static_cast<uint32_t>(g) * static_cast<uint32_t>(g);
-  for (uint32_t k = tid; k < gg; k += kThreads) A.proxy[k] = 0;
-  for (uint32_t k = tid; k < (gg + 31) / 32; k += kThreads) A.candbits[k] = 0;
-  __syncthreads();
-
-  // Phase 3: staged entries of the row views with the doubled self entry
-  // (quotient_graph.cpp:596-616) -- cross entries first, self entry last --
-  // approx_old_cost per member (sweg.cpp:1829-1850), and the target table of
-  // cross entries (buckets_by_target, sweg.cpp:1848).  Self entries never
-  // pair (sweg.cpp:1874, 1883), so they stay out of the buckets.
-  for (uint32_t e = tid; e < E; e += kThreads) {
-    const int i = UpperSegment(m_eoff, g, e);
-    const int k = static_cast<int>(e - m_eoff[i]);
-    const int rep = m_rep[i];
-    int t, cnt, st;
-    if (k < m_len[i]) {
-      const Entry en = p.q.pool[m_off[i] + k];
-      t = en.nbr;
-      cnt = en.cnt;
-      st = p.q.size[t];
-    } else {
-      t = rep;
-      cnt = 2 * m_int[i];
-      st = m_size[i];
-    }
-    A.ent_t[e] = t;
-    A.ent_e[e] = cnt;
-    A.ent_st[e] = st;
-    if (small) s_ent_i[e] = static_cast<int16_t>(i); else A.g_ent_i[e] = i;
-    atomicAdd(reinterpret_cast<unsigned long long*>(&m_aoc[i]),
-              static_cast<unsigned long long>(EncodeCost(rep, m_size[i], t, st, cnt)));
-    int sl = -1, rank = 0;
-    if (t != rep) {
-      uint32_t h = HashInt(static_cast<uint32_t>(t)) & (TC - 1);
-      while (true) {
-        const int prev = atomicCAS(&A.th_key[h], -1, t);
-        if (prev == -1 || prev == t) break;
-        h = (h + 1) & (TC - 1);
-      }
-      sl = static_cast<int>(h);
-      rank = static_cast<int>(atomicAdd(&A.th_cnt[h], 1U));
-      if (!small) A.th_st[h] = st;  // every entry of target t writes t's size
-    }
-    A.ent_slot[e] = sl;
-    if (small) s_ent_rank[e] = static_cast<int16_t>(rank); else A.g_ent_rank[e] = rank;
-  }
-  __syncthreads();
-
-  stamp(7);
-  // Phase 4: bucket offsets, scatter, pairs per bucket.
-  ScanInPlace(A.th_cnt, static_cast<int>(TC), wave_tot);
-#pragma unroll 4
-  for (uint32_t e = tid; e < E; e += kThreads) {
-    const int sl = A.ent_slot[e];
-    if (sl < 0) continue;
-    const int rank = small ? s_ent_rank[e] : A.g_ent_rank[e];
-    const uint32_t pos = A.th_cnt[sl] + static_cast<uint32_t>(rank);
-    if (small) {
-      s_bucket[pos] = static_cast<int16_t>(e);
-    } else {
-      reinterpret_cast<unsigned long long*>(A.g_bucket)[pos] =
-          (static_cast<unsigned long long>(static_cast<uint32_t>(A.g_ent_i[e])) << 32) |
-          static_cast<uint32_t>(A.ent_e[e]);
-    }
-  }
-#pragma unroll 8
-  for (uint32_t k = tid; k < TC; k += kThreads) {
-    const uint32_t s = A.th_cnt[k + 1] - A.th_cnt[k];
-    A.pair_pref[k] = s >= 2 ? s * (s - 1) / 2 : 0;
-  }
-  __syncthreads();
-  const uint32_t PT = Uniform(ScanInPlace(A.pair_pref, static_cast<int>(TC), wave_tot));
-
-  // Phase 5: bucket pair expansion (sweg.cpp:1863-1903): for two members
-  // sharing target t, delta = EC(a,t,ea) + EC(b,t,eb) - EC(a+b,t,ea+eb),
-  // accumulated only when positive.
-  // Small groups here, from LDS; large groups below.
-  for (uint32_t pi = tid; small && pi < PT; pi += kThreads) {
-    const int sl = UpperSegment(A.pair_pref, static_cast<int>(TC), pi);
-    const uint32_t l = pi - A.pair_pref[sl];
-    const uint32_t off = A.th_cnt[sl];
-    const uint32_t s = A.th_cnt[sl + 1] - off;
-    uint32_t x, y;
-    TriDecode(l, s, &x, &y);
-    const int ex = s_bucket[off + x];
-    const int ey = s_bucket[off + y];
-    const int i = s_ent_i[ex];
-    const int j = s_ent_i[ey];
-    const int t = A.ent_t[ex];
-    const int64_t st = A.ent_st[ex];
-    const int64_t ea = A.ent_e[ex], eb = A.ent_e[ey];
-    const int64_t sa = m_size[i], sb = m_size[j];
-    const int ra = m_rep[i], rb = m_rep[j];
-    const int64_t old_cost = EncodeCost(ra, sa, t, st, ea) + EncodeCost(rb, sb, t, st, eb);
-    const int64_t new_cost = EncodeCost(ra, sa + sb, t, st, ea + eb);
-    const int64_t delta = old_cost - new_cost;
-    // t is a common neighbour of members i and j: its term of their exact
-    // gain's common-neighbour correction goes into the high word.
-    const int64_t sw = sa + sb;
-    const int64_t corr = BlockCost(sw, st, ea + eb, false, p.err) - BlockCost(sw, st, ea, false, p.err) -
-                         BlockCost(sw, st, eb, false, p.err);
-    const unsigned long long add =
-        (delta > 0 ? static_cast<unsigned long long>(delta) : 0ULL) + (static_cast<unsigned long long>(corr) << 32);
-    if (add != 0) {
-      const int lo = min(i, j), hi = max(i, j);
-      atomicAdd(reinterpret_cast<unsigned long long*>(&A.proxy[lo * g + hi]), add);
-    }
-  }
-  if (!small && PT > 0) {
-    // Large groups: compact the buckets with pairs, index blocks of 2^shift
-    // pairs by their first bucket, then each pair finds its bucket with a
-    // short search in LDS and reads the bucket's packed entries: two levels
-    // of global loads per pair.
-#pragma unroll 8
-    for (uint32_t k = tid; k < TC; k += kThreads) A.nz_pos[k] = A.th_cnt[k + 1] - A.th_cnt[k] >= 2 ? 1U : 0U;
+  uint32_t PT = 0, NB = 0, nblk = 0;
+  int shift = 6;
+  if (mode == kModeBack) {
+    PT = rec->PT;
+    for (int i = tid; i < g; i += kThreads) m_aoc[i] = A.maoc[i];
     __syncthreads();
-    const uint32_t NB = Uniform(ScanInPlace(A.nz_pos, static_cast<int>(TC), wave_tot));
-    uint32_t* nzp = NB <= static_cast<uint32_t>(kNzLds) ? reinterpret_cast<uint32_t*>(s_region + kNzPrefOffset)
-                                                         : A.nz_pref;
-#pragma unroll 4
+  } else {
+    // Clear the target table, the proxy matrix and the candidate bits.
     for (uint32_t k = tid; k < TC; k += kThreads) {
-      const uint32_t off = A.th_cnt[k], sz = A.th_cnt[k + 1] - off;
-      if (sz < 2) continue;
-      const uint32_t q = A.nz_pos[k];
-      A.nz_info[q] = make_uint4(off, sz, static_cast<uint32_t>(A.th_key[k]), static_cast<uint32_t>(A.th_st[k]));
-      nzp[q] = A.pair_pref[k];
+      A.th_key[k] = -1;
+      A.th_cnt[k] = 0;
     }
-    if (tid == 0) nzp[NB] = PT;
-    __syncthreads();
-    int shift = 6;
-    while (((PT - 1) >> shift) + 1 > static_cast<uint32_t>(kBlkMax)) ++shift;
-    const uint32_t nblk = ((PT - 1) >> shift) + 1;
-    uint32_t* blk = reinterpret_cast<uint32_t*>(s_region);
-    for (uint32_t k = tid; k < NB; k += kThreads) {
-      const uint32_t pb = nzp[k], pe = nzp[k + 1];
-      for (uint32_t b = (pb + (1U << shift) - 1) >> shift; b <= ((pe - 1) >> shift); ++b) blk[b] = k;
+    const uint32_t gg = static_cast<uint32_t>(g) * static_cast<uint32_t>(g);
+    for (uint32_t k = tid; k < gg; k += kThreads) A.proxy[k] = 0;
+    if (small || mode == kModeFront) {
+      for (uint32_t k = tid; k < (gg + 31) / 32; k += kThreads) A.candbits[k] = 0;
     }
     __syncthreads();
-    // Four pairs per thread at a time: their loads are in flight together.
-    constexpr int kU = 4;
-    for (uint32_t base = tid; base < PT; base += kU * kThreads) {
-      uint32_t lo[kU], l[kU];
-      uint4 info[kU];
-#pragma unroll
-      for (int u = 0; u < kU; ++u) {
-        const uint32_t pi = min(base + u * kThreads, PT - 1);
-        const uint32_t b = pi >> shift;
-        uint32_t a = blk[b], c = b + 1 < nblk ? blk[b + 1] : NB - 1;
-        while (a < c) {  // last bucket in [a, c] whose first pair is <= pi
-          const uint32_t mid = (a + c + 1) >> 1;
-          if (nzp[mid] <= pi) a = mid; else c = mid - 1;
-        }
-        lo[u] = a;
-        l[u] = pi - nzp[a];
-      }
-#pragma unroll
-      for (int u = 0; u < kU; ++u) info[u] = A.nz_info[lo[u]];
-      unsigned long long wx[kU], wy[kU];
-#pragma unroll
-      for (int u = 0; u < kU; ++u) {
-        uint32_t x, y;
-        TriDecode(l[u], info[u].y, &x, &y);
-        const unsigned long long* be = reinterpret_cast<const unsigned long long*>(A.g_bucket) + info[u].x;
-        wx[u] = be[x];
-        wy[u] = be[y];
-      }
-#pragma unroll
-      for (int u = 0; u < kU; ++u) {
-        if (base + u * kThreads >= PT) break;
-        const int i = static_cast<int>(wx[u] >> 32), j = static_cast<int>(wy[u] >> 32);
-        const int t = static_cast<int>(info[u].z);
-        const int64_t st = static_cast<int64_t>(info[u].w);
-        const int64_t ea = static_cast<uint32_t>(wx[u]), eb = static_cast<uint32_t>(wy[u]);
-        const int64_t sa = m_size[i], sb = m_size[j];
-        const int ra = m_rep[i], rb = m_rep[j];
-        const int64_t old_cost = EncodeCost(ra, sa, t, st, ea) + EncodeCost(rb, sb, t, st, eb);
-        const int64_t new_cost = EncodeCost(ra, sa + sb, t, st, ea + eb);
-        const int64_t delta = old_cost - new_cost;
-        const int64_t sw = sa + sb;
-        const int64_t corr = BlockCost(sw, st, ea + eb, false, p.err) - BlockCost(sw, st, ea, false, p.err) -
-                             BlockCost(sw, st, eb, false, p.err);
-        const unsigned long long add = (delta > 0 ? static_cast<unsigned long long>(delta) : 0ULL) +
-                                       (static_cast<unsigned long long>(corr) << 32);
-        if (add != 0) {
-          const int a = min(i, j), c = max(i, j);
-          atomicAdd(reinterpret_cast<unsigned long long*>(&A.proxy[a * g + c]), add);
-        }
-      }
-    }
-  }
-  stamp(15);
-  // Phase 6: direct edges between members (sweg.cpp:1905-1924).
-  for (uint32_t e = tid; e < E; e += kThreads) {
-    if (A.ent_slot[e] < 0) continue;
-    const int t = A.ent_t[e];
-    uint32_t h = HashInt(static_cast<uint32_t>(t)) & (kMemberSlots - 1);
-    int j = -1;
-    while (true) {
-      const int key = mh_key[h];
-      if (key == t) {
-        j = mh_val[h];
-        break;
-      }
-      if (key == -1) break;
-      h = (h + 1) & (kMemberSlots - 1);
-    }
-    const int i = small ? s_ent_i[e] : A.g_ent_i[e];
-    if (j > i) {
-      atomicAdd(reinterpret_cast<unsigned long long*>(&A.proxy[i * g + j]),
-                static_cast<unsigned long long>(A.ent_e[e]));
-    }
-  }
-  __syncthreads();
 
-  stamp(8);
-  // Phase 7: per-member top-k by (ratio desc, proxy desc, rep_lo, rep_hi)
-  // with ratio = proxy / max(1, aoc_a + aoc_b) in FP64 (sweg.cpp:1940-1977);
-  // the union of kept pairs is recorded as bits.  Within member i's row,
-  // (rep_lo, rep_hi) = (min, max)(rep_i, rep_j) orders exactly as rep_j
-  // ascending, so the key is (ratio desc, proxy desc, rep_j asc).  A row
-  // with at most k candidates keeps them all without ranking.
-  {
-    const int wave = WaveId(), lane = LaneId();
-    const int per_lane = (g + kWave - 1) / kWave;  // <= 8
-    for (int i = wave; i < g; i += kWaves) {
-      // Keys compacted into the wave's table (two words each, partner in
-      // wj) are ranked by counting with at most two keys per lane; a key is
-      // kept when fewer than k keys beat it.
-      int16_t* wj = s_topj[WaveId()];
-      auto rank_table = [&](int n) {
-        const unsigned long long* wt =
-            reinterpret_cast<const unsigned long long*>(A.tables + WaveId() * kTableSlots * 2);
-        const bool h0 = lane < n, h1 = lane + kWave < n;
-        const unsigned long long r0 = h0 ? wt[2 * lane] : 0, k0 = h0 ? wt[2 * lane + 1] : 0;
-        const unsigned long long r1 = h1 ? wt[2 * (lane + kWave)] : 0, k1 = h1 ? wt[2 * (lane + kWave) + 1] : 0;
-        int b0 = 0, b1 = 0;
-#pragma unroll 4
-        for (int e = 0; e < n; ++e) {
-          const unsigned long long er = wt[2 * e], ep = wt[2 * e + 1];
-          b0 += (er > r0 || (er == r0 && ep > k0)) ? 1 : 0;
-          b1 += (er > r1 || (er == r1 && ep > k1)) ? 1 : 0;
-        }
-        if (h0 && b0 < topk) {
-          const int j = wj[lane];
-          const uint32_t bit = static_cast<uint32_t>(min(i, j)) * g + static_cast<uint32_t>(max(i, j));
-          atomicOr(&A.candbits[bit >> 5], 1U << (bit & 31));
-        }
-        if (h1 && b1 < topk) {
-          const int j = wj[lane + kWave];
-          const uint32_t bit = static_cast<uint32_t>(min(i, j)) * g + static_cast<uint32_t>(max(i, j));
-          atomicOr(&A.candbits[bit >> 5], 1U << (bit & 31));
-        }
-        WaveSync();
-      };
-      // The row's proxies, loaded in one batch (all in flight together).
-      long long pw[8];
-#pragma unroll
-      for (int r = 0; r < 8; ++r) {
-        const int j = lane + r * kWave;
-        pw[r] = r < per_lane && j < g && j != i ? ProxyOf(A.proxy[min(i, j) * g + max(i, j)]) : 0;
+    // Phase 3: staged entries of the row views with the doubled self entry
+    // (quotient_graph.cpp:596-616) -- cross entries first, self entry last --
+    // approx_old_cost per member (sweg.cpp:1829-1850), and the target table of
+    // cross entries (buckets_by_target, sweg.cpp:1848).  Self entries never
+    // pair (sweg.cpp:1874, 1883), so they stay out of the buckets.
+    for (uint32_t e = tid; e < E; e += kThreads) {
+      const int i = UpperSegment(m_eoff, g, e);
+      const int k = static_cast<int>(e - m_eoff[i]);
+      const int rep = m_rep[i];
+      int t, cnt, st;
+      if (k < m_len[i]) {
+        const Entry en = p.q.pool[m_off[i] + k];
+        t = en.nbr;
+        cnt = en.cnt;
+        st = p.q.size[t];
+      } else {
+        t = rep;
+        cnt = 2 * m_int[i];
+        st = m_size[i];
       }
-      int cnt = 0;
-#pragma unroll
-      for (int r = 0; r < 8; ++r) cnt += pw[r] > 0 ? 1 : 0;
-      cnt = WaveSum32(cnt);
-      if (cnt <= topk) {
+      A.ent_t[e] = t;
+      A.ent_e[e] = cnt;
+      A.ent_st[e] = st;
+      if (small) s_ent_i[e] = static_cast<int16_t>(i); else A.g_ent_i[e] = i;
+      atomicAdd(reinterpret_cast<unsigned long long*>(&m_aoc[i]),
+                static_cast<unsigned long long>(EncodeCost(rep, m_size[i], t, st, cnt)));
+      int sl = -1, rank = 0;
+      if (t != rep) {
+        uint32_t h = HashInt(static_cast<uint32_t>(t)) & (TC - 1);
+        while (true) {
+          const int prev = atomicCAS(&A.th_key[h], -1, t);
+          if (prev == -1 || prev == t) break;
+          h = (h + 1) & (TC - 1);
+        }
+        sl = static_cast<int>(h);
+        rank = static_cast<int>(atomicAdd(&A.th_cnt[h], 1U));
+        if (!small) A.th_st[h] = st;  // every entry of target t writes t's size
+      }
+      A.ent_slot[e] = sl;
+      if (small) s_ent_rank[e] = static_cast<int16_t>(rank); else A.g_ent_rank[e] = rank;
+    }
+    __syncthreads();
+
+    stamp(7);
+    // Phase 4: bucket offsets, scatter, pairs per bucket.
+    ScanInPlace(A.th_cnt, static_cast<int>(TC), wave_tot);
+#pragma unroll 4
+    for (uint32_t e = tid; e < E; e += kThreads) {
+      const int sl = A.ent_slot[e];
+      if (sl < 0) continue;
+      const int rank = small ? s_ent_rank[e] : A.g_ent_rank[e];
+      const uint32_t pos = A.th_cnt[sl] + static_cast<uint32_t>(rank);
+      if (small) {
+        s_bucket[pos] = static_cast<int16_t>(e);
+      } else {
+        reinterpret_cast<unsigned long long*>(A.g_bucket)[pos] =
+            (static_cast<unsigned long long>(static_cast<uint32_t>(A.g_ent_i[e])) << 32) |
+            static_cast<uint32_t>(A.ent_e[e]);
+      }
+    }
+#pragma unroll 8
+    for (uint32_t k = tid; k < TC; k += kThreads) {
+      const uint32_t s = A.th_cnt[k + 1] - A.th_cnt[k];
+      A.pair_pref[k] = s >= 2 ? s * (s - 1) / 2 : 0;
+    }
+    __syncthreads();
+    PT = Uniform(ScanInPlace(A.pair_pref, static_cast<int>(TC), wave_tot));
+
+    // Phase 5: bucket pair expansion (sweg.cpp:1863-1903): for two members
+    // sharing target t, delta = EC(a,t,ea) + EC(b,t,eb) - EC(a+b,t,ea+eb),
+    // accumulated only when positive.
+    // Small groups here, from LDS; large groups below.
+    for (uint32_t pi = tid; small && pi < PT; pi += kThreads) {
+      const int sl = UpperSegment(A.pair_pref, static_cast<int>(TC), pi);
+      const uint32_t l = pi - A.pair_pref[sl];
+      const uint32_t off = A.th_cnt[sl];
+      const uint32_t s = A.th_cnt[sl + 1] - off;
+      uint32_t x, y;
+      TriDecode(l, s, &x, &y);
+      const int ex = s_bucket[off + x];
+      const int ey = s_bucket[off + y];
+      const int i = s_ent_i[ex];
+      const int j = s_ent_i[ey];
+      const int t = A.ent_t[ex];
+      const int64_t st = A.ent_st[ex];
+      const int64_t ea = A.ent_e[ex], eb = A.ent_e[ey];
+      const int64_t sa = m_size[i], sb = m_size[j];
+      const int ra = m_rep[i], rb = m_rep[j];
+      const int64_t old_cost = EncodeCost(ra, sa, t, st, ea) + EncodeCost(rb, sb, t, st, eb);
+      const int64_t new_cost = EncodeCost(ra, sa + sb, t, st, ea + eb);
+      const int64_t delta = old_cost - new_cost;
+      // t is a common neighbour of members i and j: its term of their exact
+      // gain's common-neighbour correction goes into the high word.
+      const int64_t sw = sa + sb;
+      const int64_t corr = BlockCost(sw, st, ea + eb, false, p.err) - BlockCost(sw, st, ea, false, p.err) -
+                           BlockCost(sw, st, eb, false, p.err);
+      const unsigned long long add =
+          (delta > 0 ? static_cast<unsigned long long>(delta) : 0ULL) + (static_cast<unsigned long long>(corr) << 32);
+      if (add != 0) {
+        const int lo = min(i, j), hi = max(i, j);
+        atomicAdd(reinterpret_cast<unsigned long long*>(&A.proxy[lo * g + hi]), add);
+      }
+    }
+    if (!small && PT > 0) {
+      // Large groups: compact the buckets with pairs and index blocks of
+      // 2^shift pairs by their first bucket (in LDS on this workgroup, in the
+      // scratch for a front, whose pairs the pair kernel expands).
+#pragma unroll 8
+      for (uint32_t k = tid; k < TC; k += kThreads) A.nz_pos[k] = A.th_cnt[k + 1] - A.th_cnt[k] >= 2 ? 1U : 0U;
+      __syncthreads();
+      NB = Uniform(ScanInPlace(A.nz_pos, static_cast<int>(TC), wave_tot));
+      uint32_t* nzp = mode == kModeFull && NB <= static_cast<uint32_t>(kNzLds)
+                          ? reinterpret_cast<uint32_t*>(s_region + kNzPrefOffset)
+                          : A.nz_pref;
+#pragma unroll 4
+      for (uint32_t k = tid; k < TC; k += kThreads) {
+        const uint32_t off = A.th_cnt[k], sz = A.th_cnt[k + 1] - off;
+        if (sz < 2) continue;
+        const uint32_t q = A.nz_pos[k];
+        A.nz_info[q] = make_uint4(off, sz, static_cast<uint32_t>(A.th_key[k]), static_cast<uint32_t>(A.th_st[k]));
+        nzp[q] = A.pair_pref[k];
+      }
+      if (tid == 0) nzp[NB] = PT;
+      __syncthreads();
+      while (((PT - 1) >> shift) + 1 > static_cast<uint32_t>(kBlkMax)) ++shift;
+      nblk = ((PT - 1) >> shift) + 1;
+      uint32_t* blk = mode == kModeFull ? reinterpret_cast<uint32_t*>(s_region) : A.blk;
+      for (uint32_t k = tid; k < NB; k += kThreads) {
+        const uint32_t pb = nzp[k], pe = nzp[k + 1];
+        for (uint32_t b = (pb + (1U << shift) - 1) >> shift; b <= ((pe - 1) >> shift); ++b) blk[b] = k;
+      }
+      __syncthreads();
+      bool inline_pairs = mode == kModeFull;
+      if (mode == kModeFront) {
+        // Queue the pair chunks; without room they run here.
+        const int pc = static_cast<int>((PT + kPairChunk - 1) / kPairChunk);
+        if (tid == 0) {
+          const int pb = atomicAdd(&p.round->npair_items, pc);
+          s_flag = pb + pc <= p.item_cap ? 1 : 0;
+          s_pb = pb;
+        }
+        __syncthreads();
+        const int pb = s_pb;
+        const int ok = s_flag;
+        __syncthreads();
+        for (int c = tid; c < pc && pb + c < p.item_cap; c += kThreads)
+          p.pair_items[pb + c] = make_int2(ok ? static_cast<int>(rec - p.split) : -1, c);
+        inline_pairs = ok == 0;
+      }
+      if (inline_pairs) BucketPairs(A, blk, nzp, m_size, m_rep, g, PT, NB, shift, nblk, 0, PT, p.err);
+    }
+    stamp(15);
+    // Phase 6: direct edges between members (sweg.cpp:1905-1924).
+    for (uint32_t e = tid; e < E; e += kThreads) {
+      if (A.ent_slot[e] < 0) continue;
+      const int t = A.ent_t[e];
+      uint32_t h = HashInt(static_cast<uint32_t>(t)) & (kMemberSlots - 1);
+      int j = -1;
+      while (true) {
+        const int key = mh_key[h];
+        if (key == t) {
+          j = mh_val[h];
+          break;
+        }
+        if (key == -1) break;
+        h = (h + 1) & (kMemberSlots - 1);
+      }
+      const int i = small ? s_ent_i[e] : A.g_ent_i[e];
+      if (j > i) {
+        atomicAdd(reinterpret_cast<unsigned long long*>(&A.proxy[i * g + j]),
+                  static_cast<unsigned long long>(A.ent_e[e]));
+      }
+    }
+    __syncthreads();
+    if (mode == kModeFront) {
+      // The pair and top-k kernels take it from here: the members' state they
+      // need, then the record.
+      for (int i = tid; i < g; i += kThreads) {
+        A.maoc[i] = m_aoc[i];
+        A.mrep[i] = m_rep[i];
+        A.msize[i] = m_size[i];
+      }
+      if (tid == 0) {
+        rec->E = static_cast<int>(E);
+        rec->PT = PT;
+        rec->NB = NB;
+        rec->shift = shift;
+        rec->nblk = nblk;
+        rec->base = s_base;
+        rec->rows_done = 0;
+        rec->ok = 1;
+      }
+      return;
+    }
+    if (!small) {
+      // Large groups keep the kept-pair bits in the LDS region from here to
+      // the candidate listing (the bucket index there is no longer needed;
+      // g^2 / 32 words lie below the join tables).
+      A.candbits = reinterpret_cast<uint32_t*>(s_region);
+      for (uint32_t k = tid; k < (gg + 31) / 32; k += kThreads) A.candbits[k] = 0;
+      __syncthreads();
+    }
+
+    stamp(8);
+    // Phase 7: per-member top-k by (ratio desc, proxy desc, rep_lo, rep_hi)
+    // with ratio = proxy / max(1, aoc_a + aoc_b) in FP64 (sweg.cpp:1940-1977);
+    // the union of kept pairs is recorded as bits.  Within member i's row,
+    // (rep_lo, rep_hi) = (min, max)(rep_i, rep_j) orders exactly as rep_j
+    // ascending, so the key is (ratio desc, proxy desc, rep_j asc).  A row
+    // with at most k candidates keeps them all without ranking.
+    {
+      const int wave = WaveId(), lane = LaneId();
+      const int per_lane = (g + kWave - 1) / kWave;  // <= 8
+      // Row i's proxies; the next row's are loaded while this one is ranked.
+      auto load_row = [&](int ii, long long* dst) {
 #pragma unroll
         for (int r = 0; r < 8; ++r) {
           const int j = lane + r * kWave;
-          if (pw[r] > 0) {
+          dst[r] = ii < g && r < per_lane && j < g && j != ii ? ProxyOf(A.proxy[min(ii, j) * g + max(ii, j)]) : 0;
+        }
+      };
+      long long pn[8];
+      load_row(wave, pn);
+      for (int i = wave; i < g; i += kWaves) {
+        long long pw[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) pw[r] = pn[r];
+        load_row(i + kWaves, pn);
+        // Keys compacted into the wave's table (two words each, partner in
+        // wj) are ranked by counting with at most two keys per lane; a key is
+        // kept when fewer than k keys beat it.
+        int16_t* wj = s_topj[WaveId()];
+        auto rank_table = [&](int n) {
+          const unsigned long long* wt =
+              reinterpret_cast<const unsigned long long*>(A.tables + WaveId() * kTableSlots * 2);
+          const bool h0 = lane < n, h1 = lane + kWave < n;
+          const unsigned long long r0 = h0 ? wt[2 * lane] : 0, k0 = h0 ? wt[2 * lane + 1] : 0;
+          const unsigned long long r1 = h1 ? wt[2 * (lane + kWave)] : 0, k1 = h1 ? wt[2 * (lane + kWave) + 1] : 0;
+          int b0 = 0, b1 = 0;
+#pragma unroll 4
+          for (int e = 0; e < n; ++e) {
+            const unsigned long long er = wt[2 * e], ep = wt[2 * e + 1];
+            b0 += (er > r0 || (er == r0 && ep > k0)) ? 1 : 0;
+            b1 += (er > r1 || (er == r1 && ep > k1)) ? 1 : 0;
+          }
+          if (h0 && b0 < topk) {
+            const int j = wj[lane];
+            const uint32_t bit = static_cast<uint32_t>(min(i, j)) * g + static_cast<uint32_t>(max(i, j));
+            atomicOr(&A.candbits[bit >> 5], 1U << (bit & 31));
+          }
+          if (h1 && b1 < topk) {
+            const int j = wj[lane + kWave];
+            const uint32_t bit = static_cast<uint32_t>(min(i, j)) * g + static_cast<uint32_t>(max(i, j));
+            atomicOr(&A.candbits[bit >> 5], 1U << (bit & 31));
+          }
+          WaveSync();
+        };
+        int cnt = 0;
+#pragma unroll
+        for (int r = 0; r < 8; ++r) cnt += pw[r] > 0 ? 1 : 0;
+        cnt = WaveSum32(cnt);
+        if (cnt <= topk) {
+#pragma unroll
+          for (int r = 0; r < 8; ++r) {
+            const int j = lane + r * kWave;
+            if (pw[r] > 0) {
+              const uint32_t bit = static_cast<uint32_t>(min(i, j)) * g + static_cast<uint32_t>(max(i, j));
+              atomicOr(&A.candbits[bit >> 5], 1U << (bit & 31));
+            }
+          }
+          continue;
+        }
+        double ratio[8];
+        long long pv[8];
+        uint32_t live = 0;
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+          ratio[r] = 0.0;
+          pv[r] = 0;
+          const int j = lane + r * kWave;
+          {
+            const long long v = pw[r];
+            if (v > 0) {
+              long long denom = m_aoc[i] + m_aoc[j];
+              if (denom < 1) denom = 1;
+              ratio[r] = static_cast<double>(v) / static_cast<double>(denom);
+              pv[r] = v;
+              live |= 1U << r;
+            }
+          }
+        }
+        if (cnt <= kRankMax) {
+          // Rank by counting: key = (ratio bits, proxy << 32 | ~rep_j), larger
+          // is better (positive doubles order as their bit patterns; the proxy
+          // is below 2^32, see ProxyOf).  The row's
+          // keys are compacted into the wave's table; a candidate is kept when
+          // fewer than k keys beat it.
+          unsigned long long* wk = reinterpret_cast<unsigned long long*>(A.tables + WaveId() * kTableSlots * 2);
+          unsigned long long kp[8];
+          int base = 0;
+#pragma unroll
+          for (int r = 0; r < 8; ++r) {
+            kp[r] = (static_cast<unsigned long long>(pv[r]) << 32) |
+                    static_cast<unsigned long long>(0xffffffffU - static_cast<uint32_t>(m_rep[min(g - 1, lane + r * kWave)]));
+            if (r >= per_lane) continue;
+            const bool have = (live >> r) & 1U;
+            const unsigned long long mask = __ballot(have);
+            if (have) {
+              const int pos = base + __popcll(mask & ((1ULL << lane) - 1ULL));
+              wk[2 * pos] = static_cast<unsigned long long>(__double_as_longlong(ratio[r]));
+              wk[2 * pos + 1] = kp[r];
+              wj[pos] = static_cast<int16_t>(lane + r * kWave);
+            }
+            base += __popcll(mask);
+          }
+          WaveSync();
+          rank_table(cnt);
+          continue;
+        }
+        {
+          // More candidates than the table holds: only keys at least the k-th
+          // best of the lanes' best keys (k distinct candidates) can be among
+          // the row's top k, so only those are ranked.  With more than 128
+          // candidates on at most 8 per lane, at least 17 lanes hold one.
+          unsigned long long* wk = reinterpret_cast<unsigned long long*>(A.tables + WaveId() * kTableSlots * 2);
+          unsigned long long kp[8], hr = 0, hp = 0;
+#pragma unroll
+          for (int r = 0; r < 8; ++r) {
+            kp[r] = (static_cast<unsigned long long>(pv[r]) << 32) |
+                    static_cast<unsigned long long>(0xffffffffU - static_cast<uint32_t>(m_rep[min(g - 1, lane + r * kWave)]));
+            if (!((live >> r) & 1U)) continue;
+            const unsigned long long kr = static_cast<unsigned long long>(__double_as_longlong(ratio[r]));
+            if (kr > hr || (kr == hr && kp[r] > hp)) {
+              hr = kr;
+              hp = kp[r];
+            }
+          }
+          wk[2 * lane] = hr;
+          wk[2 * lane + 1] = hp;
+          WaveSync();
+          int hrank = 0;
+          #pragma unroll 4
+          for (int e = 0; e < kWave; ++e) {
+            const unsigned long long er = wk[2 * e], ep = wk[2 * e + 1];
+            hrank += (er > hr || (er == hr && ep > hp)) ? 1 : 0;
+          }
+          WaveSync();
+          const int cap = kRankMax - 1;  // the last key slot carries the threshold
+          if (lane == 0) {
+            wk[2 * cap] = 0;  // no threshold (fewer valid lanes than k): rank all, below
+            wk[2 * cap + 1] = 0;
+          }
+          WaveSync();
+          if (hr != 0 && hrank == topk - 1) {
+            wk[2 * cap] = hr;
+            wk[2 * cap + 1] = hp;
+          }
+          WaveSync();
+          const unsigned long long tr = wk[2 * cap], tp = wk[2 * cap + 1];
+          WaveSync();
+          int base = 0;
+#pragma unroll
+          for (int r = 0; r < 8; ++r) {
+            if (r >= per_lane) continue;
+            const unsigned long long kr = static_cast<unsigned long long>(__double_as_longlong(ratio[r]));
+            const bool have = ((live >> r) & 1U) && (kr > tr || (kr == tr && kp[r] >= tp));
+            const unsigned long long mask = __ballot(have);
+            if (have) {
+              const int pos = base + __popcll(mask & ((1ULL << lane) - 1ULL));
+              if (pos < cap) {
+                wk[2 * pos] = kr;
+                wk[2 * pos + 1] = kp[r];
+                wj[pos] = static_cast<int16_t>(lane + r * kWave);
+              }
+            }
+            base += __popcll(mask);
+          }
+          WaveSync();
+          if (base <= cap) {
+            rank_table(base);
+            continue;
+          }
+        }
+        for (int round = 0; round < topk; ++round) {
+          double br = 0.0;
+          long long bp = 0;
+          int brep = 0x7fffffff, bslot = -1;
+#pragma unroll
+          for (int r = 0; r < 8; ++r) {
+            if (!((live >> r) & 1U)) continue;
+            const int rep = m_rep[lane + r * kWave];
+            if (bslot < 0 || ratio[r] > br || (ratio[r] == br && (pv[r] > bp || (pv[r] == bp && rep < brep)))) {
+              br = ratio[r];
+              bp = pv[r];
+              brep = rep;
+              bslot = r;
+            }
+          }
+          int valid = bslot >= 0 ? 1 : 0;
+          int wlane = lane;
+#pragma unroll
+          for (int off = kWave / 2; off > 0; off >>= 1) {
+            const double orat = __shfl_xor(br, off, kWave);
+            const long long opv = __shfl_xor(bp, off, kWave);
+            const int orep = __shfl_xor(brep, off, kWave);
+            const int ovalid = __shfl_xor(valid, off, kWave);
+            const int olane = __shfl_xor(wlane, off, kWave);
+            const bool take = ovalid && (!valid || orat > br || (orat == br && (opv > bp || (opv == bp && orep < brep))));
+            if (take) {
+              br = orat;
+              bp = opv;
+              brep = orep;
+              valid = ovalid;
+              wlane = olane;
+            }
+          }
+          if (!valid) break;
+          if (lane == wlane) {
+            live &= ~(1U << bslot);
+            const int j = lane + bslot * kWave;
             const uint32_t bit = static_cast<uint32_t>(min(i, j)) * g + static_cast<uint32_t>(max(i, j));
             atomicOr(&A.candbits[bit >> 5], 1U << (bit & 31));
           }
         }
-        continue;
-      }
-      double ratio[8];
-      long long pv[8];
-      uint32_t live = 0;
-#pragma unroll
-      for (int r = 0; r < 8; ++r) {
-        ratio[r] = 0.0;
-        pv[r] = 0;
-        const int j = lane + r * kWave;
-        {
-          const long long v = pw[r];
-          if (v > 0) {
-            long long denom = m_aoc[i] + m_aoc[j];
-            if (denom < 1) denom = 1;
-            ratio[r] = static_cast<double>(v) / static_cast<double>(denom);
-            pv[r] = v;
-            live |= 1U << r;
-          }
-        }
-      }
-      if (cnt <= kRankMax) {
-        // Rank by counting: key = (ratio bits, proxy << 32 | ~rep_j), larger
-        // is better (positive doubles order as their bit patterns; the proxy
-        // is below 2^32, see ProxyOf).  The row's
-        // keys are compacted into the wave's table; a candidate is kept when
-        // fewer than k keys beat it.
-        unsigned long long* wk = reinterpret_cast<unsigned long long*>(A.tables + WaveId() * kTableSlots * 2);
-        unsigned long long kp[8];
-        int base = 0;
-#pragma unroll
-        for (int r = 0; r < 8; ++r) {
-          kp[r] = (static_cast<unsigned long long>(pv[r]) << 32) |
-                  static_cast<unsigned long long>(0xffffffffU - static_cast<uint32_t>(m_rep[min(g - 1, lane + r * kWave)]));
-          if (r >= per_lane) continue;
-          const bool have = (live >> r) & 1U;
-          const unsigned long long mask = __ballot(have);
-          if (have) {
-            const int pos = base + __popcll(mask & ((1ULL << lane) - 1ULL));
-            wk[2 * pos] = static_cast<unsigned long long>(__double_as_longlong(ratio[r]));
-            wk[2 * pos + 1] = kp[r];
-            wj[pos] = static_cast<int16_t>(lane + r * kWave);
-          }
-          base += __popcll(mask);
-        }
-        WaveSync();
-        rank_table(cnt);
-        continue;
-      }
-      {
-        // More candidates than the table holds: only keys at least the k-th
-        // best of the lanes' best keys (k distinct candidates) can be among
-        // the row's top k, so only those are ranked.  With more than 128
-        // candidates on at most 8 per lane, at least 17 lanes hold one.
-        unsigned long long* wk = reinterpret_cast<unsigned long long*>(A.tables + WaveId() * kTableSlots * 2);
-        unsigned long long kp[8], hr = 0, hp = 0;
-#pragma unroll
-        for (int r = 0; r < 8; ++r) {
-          kp[r] = (static_cast<unsigned long long>(pv[r]) << 32) |
-                  static_cast<unsigned long long>(0xffffffffU - static_cast<uint32_t>(m_rep[min(g - 1, lane + r * kWave)]));
-          if (!((live >> r) & 1U)) continue;
-          const unsigned long long kr = static_cast<unsigned long long>(__double_as_longlong(ratio[r]));
-          if (kr > hr || (kr == hr && kp[r] > hp)) {
-            hr = kr;
-            hp = kp[r];
-          }
-        }
-        wk[2 * lane] = hr;
-        wk[2 * lane + 1] = hp;
-        WaveSync();
-        int hrank = 0;
-        #pragma unroll 4
-        for (int e = 0; e < kWave; ++e) {
-          const unsigned long long er = wk[2 * e], ep = wk[2 * e + 1];
-          hrank += (er > hr || (er == hr && ep > hp)) ? 1 : 0;
-        }
-        WaveSync();
-        const int cap = kRankMax - 1;  // the last key slot carries the threshold
-        if (lane == 0) {
-          wk[2 * cap] = 0;  // no threshold (fewer valid lanes than k): rank all, below
-          wk[2 * cap + 1] = 0;
-        }
-        WaveSync();
-        if (hr != 0 && hrank == topk - 1) {
-          wk[2 * cap] = hr;
-          wk[2 * cap + 1] = hp;
-        }
-        WaveSync();
-        const unsigned long long tr = wk[2 * cap], tp = wk[2 * cap + 1];
-        WaveSync();
-        int base = 0;
-#pragma unroll
-        for (int r = 0; r < 8; ++r) {
-          if (r >= per_lane) continue;
-          const unsigned long long kr = static_cast<unsigned long long>(__double_as_longlong(ratio[r]));
-          const bool have = ((live >> r) & 1U) && (kr > tr || (kr == tr && kp[r] >= tp));
-          const unsigned long long mask = __ballot(have);
-          if (have) {
-            const int pos = base + __popcll(mask & ((1ULL << lane) - 1ULL));
-            if (pos < cap) {
-              wk[2 * pos] = kr;
-              wk[2 * pos + 1] = kp[r];
-              wj[pos] = static_cast<int16_t>(lane + r * kWave);
-            }
-          }
-          base += __popcll(mask);
-        }
-        WaveSync();
-        if (base <= cap) {
-          rank_table(base);
-          continue;
-        }
-      }
-      for (int round = 0; round < topk; ++round) {
-        double br = 0.0;
-        long long bp = 0;
-        int brep = 0x7fffffff, bslot = -1;
-#pragma unroll
-        for (int r = 0; r < 8; ++r) {
-          if (!((live >> r) & 1U)) continue;
-          const int rep = m_rep[lane + r * kWave];
-          if (bslot < 0 || ratio[r] > br || (ratio[r] == br && (pv[r] > bp || (pv[r] == bp && rep < brep)))) {
-            br = ratio[r];
-            bp = pv[r];
-            brep = rep;
-            bslot = r;
-          }
-        }
-        int valid = bslot >= 0 ? 1 : 0;
-        int wlane = lane;
-#pragma unroll
-        for (int off = kWave / 2; off > 0; off >>= 1) {
-          const double orat = __shfl_xor(br, off, kWave);
-          const long long opv = __shfl_xor(bp, off, kWave);
-          const int orep = __shfl_xor(brep, off, kWave);
-          const int ovalid = __shfl_xor(valid, off, kWave);
-          const int olane = __shfl_xor(wlane, off, kWave);
-          const bool take = ovalid && (!valid || orat > br || (orat == br && (opv > bp || (opv == bp && orep < brep))));
-          if (take) {
-            br = orat;
-            bp = opv;
-            brep = orep;
-            valid = ovalid;
-            wlane = olane;
-          }
-        }
-        if (!valid) break;
-        if (lane == wlane) {
-          live &= ~(1U << bslot);
-          const int j = lane + bslot * kWave;
-          const uint32_t bit = static_cast<uint32_t>(min(i, j)) * g + static_cast<uint32_t>(max(i, j));
-          atomicOr(&A.candbits[bit >> 5], 1U << (bit & 31));
-        }
       }
     }
+    __syncthreads();
   }
-  __syncthreads();
 
   stamp(9);
   // Phase 8: candidate pairs in ascending (a_idx, b_idx) order
@@ -1291,11 +1633,110 @@ __device__ __forceinline__ void MergeGroupsBody(const MergeParams& p) {
       if (gidx >= p.ngroups) break;
       sel_out = p.sel + 2LL * slot * kMaxSelect;
     }
-    ScoreGroup<kRanker>(p, gidx, slot, sel_out, item, s_region);
+    ScoreGroup<kRanker>(p, gidx, slot, sel_out, item, s_region, kModeFull, nullptr);
   }
 }
 
 __global__ __launch_bounds__(kThreads) void MergeGroupsKernel(MergeParams p) { MergeGroupsBody<false>(p); }
 __global__ __launch_bounds__(kThreads) void MergeGroupsRankerKernel(MergeParams p) { MergeGroupsBody<true>(p); }
+
+namespace {
+
+// Arrays of split record r's group (the scratch layout its front allocated).
+__device__ __forceinline__ int SplitArrays(const MergeParams& p, const SplitRec& r, GroupArrays* A, uint32_t* TC) {
+  const int g = p.glen[r.gidx];
+  const uint32_t E = static_cast<uint32_t>(r.E);
+  uint32_t tc = 64;
+  while (tc < 2 * E) tc <<= 1;
+  const int topk = max(1, min(p.top_k, g - 1));
+  const uint32_t cmax = min(static_cast<uint32_t>(g) * static_cast<uint32_t>(topk),
+                            static_cast<uint32_t>(g) * static_cast<uint32_t>(g - 1) / 2);
+  LargeLayout(p.scratch + r.base, E, tc, g, cmax, *A);
+  *TC = tc;
+  return g;
+}
+
+__device__ __forceinline__ void FenceAcquireAgent() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
+// This wave's outstanding stores and atomics performed (s_waitcnt vmcnt(0)).
+__device__ __forceinline__ void WaitWrites() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+// Rows of split groups, kWaves per item; the workgroup that finishes a
+// group's last row chunk then lists, scores and selects the group.  The
+// finished-chunk count is a returning atomic after every wave's bit updates
+// are performed; the last finisher invalidates its caches (agent acquire)
+// before reading the bits.
+template <bool kRanker>
+__device__ __forceinline__ void SplitRowsBody(const MergeParams& p) {
+  __shared__ __align__(16) unsigned char s_region[kRegionBytes];
+  __shared__ int s_t, s_last;
+  const int n = min(p.round->nrow_items, p.item_cap);
+  int16_t* wj = reinterpret_cast<int16_t*>(s_region) + WaveId() * kRankMax;  // below the join tables
+  int32_t* tables = reinterpret_cast<int32_t*>(s_region + kTableOffset) + WaveId() * kTableSlots * 2;
+  for (;;) {
+    if (threadIdx.x == 0) s_t = atomicAdd(&p.round->steal_rows, 1);
+    __syncthreads();
+    const int t = s_t;
+    __syncthreads();
+    if (t >= n) break;
+    const int2 it = p.row_items[t];
+    if (it.x < 0) continue;
+    SplitRec* rec = p.split + it.x;
+    if (rec->ok == 0) continue;
+    GroupArrays A;
+    uint32_t TC;
+    const int g = Uniform(SplitArrays(p, *rec, &A, &TC));
+    const int topk = max(1, min(p.top_k, g - 1));
+    const int i = it.y * kWaves + WaveId();
+    if (i < g) {
+      const int lane = LaneId();
+      const int per_lane = (g + kWave - 1) / kWave;
+      long long pw[8];
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        const int j = lane + r * kWave;
+        pw[r] = r < per_lane && j < g && j != i ? ProxyOf(A.proxy[min(i, j) * g + max(i, j)]) : 0;
+      }
+      TopkRowRank(i, g, topk, per_lane, pw, A.maoc, A.mrep, A.candbits, tables, wj);
+    }
+    WaitWrites();
+    __syncthreads();
+    if (threadIdx.x == 0) s_last = atomicAdd(&rec->rows_done, 1) == rec->row_chunks - 1 ? 1 : 0;
+    __syncthreads();
+    if (s_last) {
+      FenceAcquireAgent();
+      __syncthreads();
+      ScoreGroup<kRanker>(p, rec->gidx, rec->gidx, p.sel + p.gstart[rec->gidx], rec->item, s_region, kModeBack, rec);
+      __syncthreads();
+    }
+  }
+}
+
+}  // namespace
+
+// Bucket-pair chunks of split groups (kPairChunk pairs per item).
+__global__ __launch_bounds__(kThreads) void SplitPairsKernel(MergeParams p) {
+  __shared__ int s_t;
+  const int n = min(p.round->npair_items, p.item_cap);
+  for (;;) {
+    if (threadIdx.x == 0) s_t = atomicAdd(&p.round->steal_pairs, 1);
+    __syncthreads();
+    const int t = s_t;
+    __syncthreads();
+    if (t >= n) break;
+    const int2 it = p.pair_items[t];
+    if (it.x < 0) continue;
+    const SplitRec& rec = p.split[it.x];
+    if (rec.ok == 0) continue;
+    GroupArrays A;
+    uint32_t TC;
+    const int g = Uniform(SplitArrays(p, rec, &A, &TC));
+    const uint32_t p0 = static_cast<uint32_t>(it.y) * kPairChunk;
+    BucketPairs(A, A.blk, A.nz_pref, A.msize, A.mrep, g, rec.PT, rec.NB, rec.shift, rec.nblk, p0,
+                min(rec.PT, p0 + kPairChunk), p.err);
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void SplitRowsKernel(MergeParams p) { SplitRowsBody<false>(p); }
+__global__ __launch_bounds__(kThreads) void SplitRowsRankerKernel(MergeParams p) { SplitRowsBody<true>(p); }
 
 }  // namespace beam
