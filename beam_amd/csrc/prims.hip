@@ -142,13 +142,26 @@ __global__ void KeyBits(const SortRec* in, int n, unsigned long long* acc) {
     a &= in[i].key;
     s |= in[i].seg;
   }
-  // One atomic per wave.
+  // One atomic per block (atomics on one address serialize).
   for (int off = kWave / 2; off > 0; off >>= 1) {
     o |= __shfl_xor(o, off, kWave);
     a &= __shfl_xor(a, off, kWave);
     s |= __shfl_xor(s, off, kWave);
   }
+  __shared__ unsigned long long w[3][256 / kWave];
+  const int wave = static_cast<int>(threadIdx.x) / kWave;
   if ((threadIdx.x & (kWave - 1)) == 0) {
+    w[0][wave] = o;
+    w[1][wave] = a;
+    w[2][wave] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < static_cast<int>(blockDim.x) / kWave; ++k) {
+      o |= w[0][k];
+      a &= w[1][k];
+      s |= w[2][k];
+    }
     atomicOr(&acc[0], o);
     atomicAnd(&acc[1], a);
     atomicOr(&acc[2], s);
@@ -191,7 +204,7 @@ SortRec* SegmentedKeySort(SortRec* recs, SortRec* tmp, int n, SortTemp& st, hipS
   st.Ensure(static_cast<size_t>(tiles) * 256);
   unsigned long long init[3] = {0ULL, ~0ULL, 0ULL};
   HIP_CHECK(hipMemcpyAsync(st.bits, init, sizeof(init), hipMemcpyHostToDevice, stream));
-  KeyBits<<<min(1024, (n + 255) / 256), 256, 0, stream>>>(recs, n, st.bits);
+  KeyBits<<<min(256, (n + 255) / 256), 256, 0, stream>>>(recs, n, st.bits);
   unsigned long long bits[3];
   HIP_CHECK(hipMemcpyAsync(bits, st.bits, sizeof(bits), hipMemcpyDeviceToHost, stream));
   HIP_CHECK(hipStreamSynchronize(stream));
