@@ -736,7 +736,7 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
     // approx_old_cost per member (sweg.cpp:1829-1850), and the target table of
     // cross entries (buckets_by_target, sweg.cpp:1848).  Self entries never
     // pair (sweg.cpp:1874, 1883), so they stay out of the buckets.
-    for (uint32_t e = tid; e < E; e += kThreads) {
+    for (uint32_t e = tid; small && e < E; e += kThreads) {
       const int i = UpperSegment(m_eoff, g, e);
       const int k = static_cast<int>(e - m_eoff[i]);
       const int rep = m_rep[i];
@@ -771,6 +771,71 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
       }
       A.ent_slot[e] = sl;
       if (small) s_ent_rank[e] = static_cast<int16_t>(rank); else A.g_ent_rank[e] = rank;
+    }
+    // Large groups: two entries per thread at a time, their loads and hash
+    // insertions in flight together.
+    for (uint32_t e0 = tid; !small && e0 < E; e0 += 2 * kThreads) {
+      int t[2], cnt[2], st[2], ii[2], sl[2] = {-1, -1}, rank[2] = {0, 0};
+      uint32_t h[2];
+      bool self[2], valid[2], pend[2];
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const uint32_t e = min(e0 + u * kThreads, E - 1);
+        valid[u] = e0 + u * kThreads < E;
+        const int i = UpperSegment(m_eoff, g, e);
+        const int k = static_cast<int>(e - m_eoff[i]);
+        ii[u] = i;
+        self[u] = k >= m_len[i];
+        if (!self[u]) {
+          const Entry en = p.q.pool[m_off[i] + k];
+          t[u] = en.nbr;
+          cnt[u] = en.cnt;
+        } else {
+          t[u] = m_rep[i];
+          cnt[u] = 2 * m_int[i];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 2; ++u) st[u] = self[u] ? m_size[ii[u]] : p.q.size[t[u]];
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        pend[u] = valid[u] && !self[u];
+        h[u] = HashInt(static_cast<uint32_t>(t[u])) & (TC - 1);
+        if (!valid[u]) continue;
+        const uint32_t e = e0 + u * kThreads;
+        A.ent_t[e] = t[u];
+        A.ent_e[e] = cnt[u];
+        A.ent_st[e] = st[u];
+        A.g_ent_i[e] = ii[u];
+        atomicAdd(reinterpret_cast<unsigned long long*>(&m_aoc[ii[u]]),
+                  static_cast<unsigned long long>(EncodeCost(m_rep[ii[u]], m_size[ii[u]], t[u], st[u], cnt[u])));
+      }
+      while (pend[0] || pend[1]) {
+        int prev[2] = {0, 0};
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          if (pend[u]) prev[u] = atomicCAS(&A.th_key[h[u]], -1, t[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          if (!pend[u]) continue;
+          if (prev[u] == -1 || prev[u] == t[u]) pend[u] = false; else h[u] = (h[u] + 1) & (TC - 1);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        if (!valid[u] || self[u]) continue;
+        sl[u] = static_cast<int>(h[u]);
+        rank[u] = static_cast<int>(atomicAdd(&A.th_cnt[h[u]], 1U));
+        A.th_st[h[u]] = st[u];  // every entry of target t writes t's size
+      }
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        if (!valid[u]) continue;
+        const uint32_t e = e0 + u * kThreads;
+        A.ent_slot[e] = sl[u];
+        A.g_ent_rank[e] = rank[u];
+      }
     }
     __syncthreads();
 
