@@ -72,7 +72,8 @@ struct GroupArrays {
   long long* cand_before;
   uint32_t* pass;       // survivor flags, then their prefix (C + 1)
   uint32_t* sig;        // ranker: 64 one-byte min-hash bins per member (16 words)
-  uint32_t* spec;       // large groups: special entries of each member (scoring)
+  uint32_t* spec;       // large groups: special entries of each member (scoring), as uint4
+                        // (target, count, target size, 1 if the target is a member)
   int32_t* tables;      // per-wave join tables (always LDS)
   // Global-mode companions of the int16 LDS arrays.
   int32_t* g_ent_i;
@@ -157,8 +158,8 @@ __device__ __forceinline__ long long CorrOf(long long w) { return w >> 32; }
 __device__ __forceinline__ unsigned long long LargeBytes(uint32_t E, uint32_t TC, int g, uint32_t cmax) {
   const unsigned long long e64 = E, tc64 = TC, g64 = static_cast<unsigned long long>(g), c64 = cmax;
   return e64 * 4 * 8 + tc64 * 4 + (tc64 + 1) * 8 + tc64 * 4 + (tc64 + 1) * 4 + 16 + tc64 * 16 + (tc64 + 1) * 4 + 16 +
-         g64 * g64 * 8 + (g64 * g64 / 32 + 1) * 4 + 16 + c64 * (8 + 8 + 2 + 2) + (c64 + 1) * 4 + g64 * 64 + e64 * 4 +
-         64 + (kBlkMaxBytes + 16) + g64 * 16 + 16;
+         g64 * g64 * 8 + (g64 * g64 / 32 + 1) * 4 + 16 + c64 * (8 + 8 + 2 + 2) + (c64 + 1) * 4 + g64 * 64 + 16 +
+         e64 * 16 + 64 + (kBlkMaxBytes + 16) + g64 * 16 + 16;
 }
 
 __device__ __forceinline__ void LargeLayout(unsigned char* r, uint32_t E, uint32_t TC, int g, uint32_t cmax,
@@ -192,7 +193,8 @@ __device__ __forceinline__ void LargeLayout(unsigned char* r, uint32_t E, uint32
   A.cand_b = reinterpret_cast<uint16_t*>(r); r += cmax * 2ULL;
   r = align8(r);
   A.sig = reinterpret_cast<uint32_t*>(r); r += static_cast<unsigned long long>(g) * 64;
-  A.spec = reinterpret_cast<uint32_t*>(r); r += E * 4ULL + 64;
+  r = reinterpret_cast<unsigned char*>((reinterpret_cast<uintptr_t>(r) + 15) & ~uintptr_t(15));
+  A.spec = reinterpret_cast<uint32_t*>(r); r += E * 16ULL + 64;  // large: uint4 per special entry
   r = align8(r);
   A.blk = reinterpret_cast<uint32_t*>(r); r += kBlkMaxBytes + 16;
   r = align8(r);
@@ -1228,33 +1230,44 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
   stamp(9);
   // Phase 8: candidate pairs in ascending (a_idx, b_idx) order
   // (sweg.cpp:1979-1989).
-  for (int lo = tid; lo < g; lo += kThreads) {
-    uint32_t cnt = 0;
-    const uint32_t b0 = static_cast<uint32_t>(lo) * g + lo + 1, b1 = static_cast<uint32_t>(lo + 1) * g;
-    for (uint32_t bit = b0; bit < b1;) {
-      const uint32_t word = A.candbits[bit >> 5] >> (bit & 31);
-      const uint32_t take = min(32U - (bit & 31), b1 - bit);
-      const uint32_t m = take == 32 ? word : (word & ((1U << take) - 1U));
-      cnt += __popc(m);
-      bit += take;
+  // One wave per row: lane l tests partner lo + 1 + 64 r + l; ballots count
+  // and place the kept partners in ascending order.
+  {
+    const int lane = LaneId();
+    for (int lo = WaveId(); lo < g; lo += kWaves) {
+      uint32_t cnt = 0;
+      for (int hb = lo + 1; hb < g; hb += kWave) {
+        const int hi = hb + lane;
+        const uint32_t bit = static_cast<uint32_t>(lo) * g + static_cast<uint32_t>(hi);
+        const bool kept = hi < g && ((A.candbits[bit >> 5] >> (bit & 31)) & 1U);
+        cnt += static_cast<uint32_t>(__popcll(__ballot(kept)));
+      }
+      if (lane == 0) m_rowcnt[lo] = cnt;
     }
-    m_rowcnt[lo] = cnt;
+    __syncthreads();
   }
-  __syncthreads();
   uint32_t C = Uniform(ScanInPlace(m_rowcnt, g, wave_tot));
   const uint32_t C0 = C;
-  for (int lo = tid; lo < g; lo += kThreads) {
-    uint32_t pos = m_rowcnt[lo];
-    for (int hi = lo + 1; hi < g; ++hi) {
-      const uint32_t bit = static_cast<uint32_t>(lo) * g + hi;
-      if ((A.candbits[bit >> 5] >> (bit & 31)) & 1U) {
-        A.cand_a[pos] = static_cast<uint16_t>(lo);
-        A.cand_b[pos] = static_cast<uint16_t>(hi);
-        ++pos;
+  {
+    const int lane = LaneId();
+    const unsigned long long below = (1ULL << lane) - 1ULL;
+    for (int lo = WaveId(); lo < g; lo += kWaves) {
+      uint32_t pos = m_rowcnt[lo];
+      for (int hb = lo + 1; hb < g; hb += kWave) {
+        const int hi = hb + lane;
+        const uint32_t bit = static_cast<uint32_t>(lo) * g + static_cast<uint32_t>(hi);
+        const bool kept = hi < g && ((A.candbits[bit >> 5] >> (bit & 31)) & 1U);
+        const unsigned long long mask = __ballot(kept);
+        if (kept) {
+          const uint32_t q = pos + static_cast<uint32_t>(__popcll(mask & below));
+          A.cand_a[q] = static_cast<uint16_t>(lo);
+          A.cand_b[q] = static_cast<uint16_t>(hi);
+        }
+        pos += static_cast<uint32_t>(__popcll(mask));
       }
     }
+    __syncthreads();
   }
-  __syncthreads();
 
   // Learned ranker: features of a candidate pair from the staged group.
   const FeatureSource fsrc{m_size, m_len, m_int, m_aoc, A.proxy, A.sig, g, p.theta};
@@ -1373,13 +1386,14 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
     long long* m_bden = m_ssp + kMaxGroup;
     uint32_t* m_nspec = reinterpret_cast<uint32_t*>(m_bden + kMaxGroup);
     uint16_t* spec16 = reinterpret_cast<uint16_t*>(m_nspec + kMaxGroup);
-    constexpr uint32_t kMemberFlag16 = 0x8000U, kMemberFlag32 = 0x80000000U;
+    constexpr uint32_t kMemberFlag16 = 0x8000U;
     for (int i = tid; i < g; i += kThreads) {
       m_ssp[i] = 0;
       m_bden[i] = 0;
       m_nspec[i] = 0;
     }
     __syncthreads();
+#pragma unroll 4
     for (uint32_t e = tid; e < E; e += kThreads) {
       if (A.ent_slot[e] < 0) continue;  // the doubled self entry is not a cross entry
       const int i = small ? s_ent_i[e] : A.g_ent_i[e];
@@ -1401,7 +1415,8 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
         if (small) {
           spec16[pos] = static_cast<uint16_t>(e | (member ? kMemberFlag16 : 0U));
         } else {
-          A.spec[pos] = e | (member ? kMemberFlag32 : 0U);
+          reinterpret_cast<uint4*>(A.spec)[pos] = make_uint4(static_cast<uint32_t>(t), static_cast<uint32_t>(ee),
+                                                             static_cast<uint32_t>(st), member ? 1U : 0U);
         }
         if (!member) {
           atomicAdd(reinterpret_cast<unsigned long long*>(&m_bden[i]),
@@ -1424,12 +1439,26 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
         const int other_rep = m_rep[side == 0 ? b : a];
         const int64_t s_me = side == 0 ? sa : sb;
         const uint32_t base = m_eoff[me], cnt = m_nspec[me];
+        const uint4* spec4 = small ? nullptr : reinterpret_cast<const uint4*>(A.spec) + base;
+#pragma unroll 4
         for (uint32_t k = 0; k < cnt; ++k) {
-          const uint32_t v = small ? spec16[base + k] : A.spec[base + k];
-          const bool member = small ? (v & kMemberFlag16) != 0 : (v & kMemberFlag32) != 0;
-          const uint32_t e = small ? (v & ~kMemberFlag16) : (v & ~kMemberFlag32);
-          const int t = A.ent_t[e];
-          const int64_t ee = A.ent_e[e], st = A.ent_st[e];
+          int t;
+          int64_t ee, st;
+          bool member;
+          if (small) {
+            const uint32_t v = spec16[base + k];
+            member = (v & kMemberFlag16) != 0;
+            const uint32_t e = v & ~kMemberFlag16;
+            t = A.ent_t[e];
+            ee = A.ent_e[e];
+            st = A.ent_st[e];
+          } else {
+            const uint4 w = spec4[k];
+            t = static_cast<int>(w.x);
+            ee = static_cast<int64_t>(w.y);
+            st = static_cast<int64_t>(w.z);
+            member = w.w != 0;
+          }
           if (member) {
             if (t == other_rep) {
               eab[side] = ee;
