@@ -7,6 +7,13 @@
 //        --candidate-index legacy --certification off --commit-policy g0
 //        --ea-use-threshold --threshold-policy reciprocal [--iterations 20]
 //        [--top-k 16] [--group-batch-size 64] [--seed S] [--write-output --out DIR]
+//        [--learned-ranker]
+//
+// --learned-ranker (no reference counterpart) turns on the learned pair
+// ranker: candidates are pruned before exact scoring after the first
+// iterations, every commit is still costed exactly and an audit falls back
+// to plain scoring; the output stays lossless but need not equal the
+// reference's.
 #include <algorithm>
 #include <chrono>
 #include <cstdint>
@@ -31,7 +38,7 @@ struct Cli {
               quotient_update = "incremental", index = "legacy", certification = "off", commit = "g0",
               threshold = "reciprocal";
   int top_k = 16, group_batch = 64, iterations = 20, print_offset = 1, hash_dims = 16, max_group = 512;
-  bool ea_threshold = false, write_output = false, help = false;
+  bool ea_threshold = false, write_output = false, help = false, ranker = false;
   double error_bound = 0.0;
   unsigned long long seed = 1;
 };
@@ -40,7 +47,9 @@ void Usage() {
   std::cout << "Usage:\n  beam --input <undirected-single-edge-list> [options]\n\n"
             << "Runs BEAM's batch-ea-blocked merge with the MAGS-compatible objective,\n"
             << "persistent quotient state, legacy candidate index, certification off and\n"
-            << "g0 commits on one MI355X (--scoring-backend hip).\n";
+            << "g0 commits on one MI355X (--scoring-backend hip).  --learned-ranker\n"
+            << "prunes candidates with the learned ranker (lossless, audited, with an\n"
+            << "exact fallback; not a reference option).\n";
 }
 
 Cli Parse(int argc, char** argv) {  // main.cpp:327-423
@@ -74,6 +83,7 @@ Cli Parse(int argc, char** argv) {  // main.cpp:327-423
     else if (a == "--out") c.out_dir = need(i);
     else if (a == "--ea-use-threshold") c.ea_threshold = true;
     else if (a == "--write-output") c.write_output = true;
+    else if (a == "--learned-ranker") c.ranker = true;
     else throw std::runtime_error("Unknown or incomplete argument: " + a);
   }
   return c;
@@ -217,7 +227,8 @@ int main(int argc, char** argv) {
               << "CUDA scoring build: " << beam_hip_build_mode() << "\n"
               << (c.write_output ? "Output directory: " + c.out_dir : std::string("Output writing: disabled")) << "\n";
 
-    beam_hip_options o{c.top_k, c.group_batch, c.hash_dims, c.max_group, 1, 0, c.seed, 0, 5, 0.25f, 0.9f};
+    beam_hip_options o{c.top_k, c.group_batch, c.hash_dims, c.max_group, 1, 0, c.seed, c.ranker ? 1 : 0,
+                       5, 0.25f, 0.9f};
     beam_hip_engine* e = nullptr;
     Check(beam_hip_create(reinterpret_cast<const int64_t*>(g.row_ptr.data()), g.col.data(), g.n, g.arcs, &o, &e),
           "beam_hip_create");
