@@ -29,6 +29,7 @@ enum : int {
   kErrPool = 8,            // quotient entry pool exhausted
   kErrSpecials = 16,       // too many merge endpoints in one neighbour row
   kErrSelection = 32,      // group selection overflowed its slot
+  kErrPairs = 64,          // a group's bucket pairs exceed 2^32 - 1
 };
 
 // Quotient row entry: neighbour supernode and edge count.

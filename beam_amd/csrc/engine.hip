@@ -692,7 +692,7 @@ void Engine::Create(const int64_t* h_row_ptr, const int32_t* h_col, int n_, long
   naff = DevAlloc<int32_t>(2);
   HIP_CHECK(hipMemsetAsync(naff, 0, 2 * sizeof(int32_t), stream));
   const unsigned long long work = static_cast<unsigned long long>(arcs) + static_cast<unsigned long long>(n);
-  scratch_a_cap = std::min<unsigned long long>(std::max<unsigned long long>(48ULL * work + (128ULL << 20), 256ULL << 20), 8ULL << 30);
+  scratch_a_cap = std::min<unsigned long long>(std::max<unsigned long long>(48ULL * work + (128ULL << 20), 256ULL << 20), 32ULL << 30);
   scratch_u_cap = std::min<unsigned long long>(std::max<unsigned long long>(16ULL * work + (64ULL << 20), 128ULL << 20), 4ULL << 30);
   scratch_a = DevAlloc<unsigned char>(scratch_a_cap);
   scratch_u = DevAlloc<unsigned char>(scratch_u_cap);
@@ -776,6 +776,7 @@ void Engine::CheckDeviceError(const char* where) {
   if (e & kErrPool) msg += " quotient entry pool exhausted;";
   if (e & kErrSpecials) msg += " too many merge endpoints in one row;";
   if (e & kErrSelection) msg += " group larger than 512;";
+  if (e & kErrPairs) msg += " a group's bucket pairs exceed 2^32 - 1;";
   throw std::runtime_error(msg);
 }
 

@@ -633,6 +633,7 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
   __shared__ uint32_t s_prefix, s_need;
   __shared__ int s_pruned;
   __shared__ unsigned long long s_best;
+  __shared__ unsigned long long s_pt64;
   __shared__ int16_t s_topj[kWaves][kRankMax];
 
   // Phase 1: members, member lookup table, staged entry offsets.
@@ -843,6 +844,7 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
 
     stamp(7);
     // Phase 4: bucket offsets, scatter, pairs per bucket.
+    if (tid == 0) s_pt64 = 0;
     ScanInPlace(A.th_cnt, static_cast<int>(TC), wave_tot);
 #pragma unroll 4
     for (uint32_t e = tid; e < E; e += kThreads) {
@@ -858,12 +860,19 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
             static_cast<uint32_t>(A.ent_e[e]);
       }
     }
+    unsigned long long pairs64 = 0;  // bucket sizes are at most g, so each count fits 32 bits
 #pragma unroll 8
     for (uint32_t k = tid; k < TC; k += kThreads) {
       const uint32_t s = A.th_cnt[k + 1] - A.th_cnt[k];
       A.pair_pref[k] = s >= 2 ? s * (s - 1) / 2 : 0;
+      pairs64 += s >= 2 ? s * (s - 1) / 2 : 0;
     }
+    if (!small) atomicAdd(&s_pt64, pairs64);
     __syncthreads();
+    if (!small && s_pt64 > 0xffffffffULL) {  // the 32-bit pair indexing would wrap
+      if (tid == 0) atomicOr(p.err, kErrPairs);
+      return;
+    }
     PT = Uniform(ScanInPlace(A.pair_pref, static_cast<int>(TC), wave_tot));
 
     // Phase 5: bucket pair expansion (sweg.cpp:1863-1903): for two members

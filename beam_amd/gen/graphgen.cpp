@@ -6,6 +6,9 @@
 // (u, v) with u < v.
 //
 //   graphgen <workload> <out.txt>     workload in {fa, am, yo, lj, u2, tiny<N>}
+//   graphgen <workload> <out.bin>     the same edges in the binary edge-list
+//                                     format the loader also reads: "BEAMEDG1",
+//                                     uint64 edge count, then uint32 (u, v) pairs
 //
 // Choices that the workload descriptions leave open are fixed here and stated
 // next to each generator (max-degree handling, community assignment, isolate
@@ -19,6 +22,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -167,6 +171,24 @@ double PowerLaw(Rng& rng, double tau, double lo, double hi) {
   return std::pow(std::pow(lo, a) + u * (std::pow(hi, a) - std::pow(lo, a)), 1.0 / a);
 }
 
+void WriteEdgesBinary(const std::vector<uint64_t>& keys, const char* path) {
+  FILE* f = std::fopen(path, "wb");
+  if (!f) throw std::runtime_error(std::string("cannot open ") + path);
+  const uint64_t m = keys.size();
+  bool ok = std::fwrite("BEAMEDG1", 1, 8, f) == 8 && std::fwrite(&m, sizeof(m), 1, f) == 1;
+  std::vector<uint32_t> buf;
+  for (uint64_t i = 0; ok && i < m; i += 1 << 20) {
+    const uint64_t e = std::min<uint64_t>(m, i + (1 << 20));
+    buf.resize(2 * (e - i));
+    for (uint64_t k = i; k < e; ++k) {
+      buf[2 * (k - i)] = static_cast<uint32_t>(keys[k] >> 32);
+      buf[2 * (k - i) + 1] = static_cast<uint32_t>(keys[k] & 0xffffffffU);
+    }
+    ok = std::fwrite(buf.data(), sizeof(uint32_t), buf.size(), f) == buf.size();
+  }
+  if (std::fclose(f) != 0 || !ok) throw std::runtime_error(std::string("cannot write ") + path);
+}
+
 void WriteEdges(const std::vector<uint64_t>& keys, const char* path) {
   FILE* f = std::fopen(path, "w");
   if (!f) throw std::runtime_error(std::string("cannot open ") + path);
@@ -310,46 +332,110 @@ std::vector<uint64_t> ChungLuOverlapping(uint64_t n, uint64_t m, double gamma, d
   return set.Sorted();
 }
 
-// R-MAT at the given scale with (a, b, c, d); edges are symmetrised (stored
-// once as lo<hi), deduplicated and self-loop free; drawing stops at
-// `target_edges` unique edges.  Isolated ids are then dropped by relabelling
-// the remaining ids in ascending order to 0..n-1.
+unsigned Threads() { return std::max(1U, std::min(16U, std::thread::hardware_concurrency())); }
+
+// Sorts v with T threads: slices sorted in parallel, then merged pairwise.
+void ParallelSort(std::vector<uint64_t>& v) {
+  const unsigned T = Threads();
+  const size_t n = v.size();
+  std::vector<size_t> cut(T + 1);
+  for (unsigned t = 0; t <= T; ++t) cut[t] = n * t / T;
+  std::vector<std::thread> ts;
+  for (unsigned t = 0; t < T; ++t) ts.emplace_back([&, t] { std::sort(v.begin() + cut[t], v.begin() + cut[t + 1]); });
+  for (auto& th : ts) th.join();
+  for (unsigned w = 1; w < T; w *= 2) {
+    ts.clear();
+    for (unsigned t = 0; t + w < T; t += 2 * w) {
+      const size_t lo = cut[t], mid = cut[t + w], hi = cut[std::min(T, t + 2 * w)];
+      ts.emplace_back([&v, lo, mid, hi] { std::inplace_merge(v.begin() + lo, v.begin() + mid, v.begin() + hi); });
+    }
+    for (auto& th : ts) th.join();
+  }
+}
+
+// R-MAT at the given scale with (a, b, c, d).  Edges are drawn in chunks of
+// 2^20 draws; chunk k uses its own generator seeded from (seed, k), so the
+// chunks can be drawn in parallel and the graph does not depend on the
+// thread count.  Draws are symmetrised (lo < hi), self-loops dropped, and
+// batches of chunks are drawn until at least `target_edges` distinct edges
+// exist (the last batch is sized from the previous batches' distinct rate, so
+// the count overshoots by well under one chunk's worth).  Isolated ids are
+// then dropped by relabelling the remaining ids in ascending order to
+// 0..n-1.  Progress goes to stderr.
 std::vector<uint64_t> Rmat(int scale, double a, double b, double c, uint64_t target_edges,
                            uint64_t seed) {
-  Rng rng(seed);
-  EdgeSet set(target_edges);
+  constexpr uint64_t kChunk = 1ULL << 20;
   const double ab = a + b, abc = a + b + c;
-  while (set.count < target_edges) {
-    uint32_t u = 0, v = 0;
-    for (int bit = scale - 1; bit >= 0; --bit) {
-      const double r = rng.Uniform();
-      if (r < a) {
-      } else if (r < ab) {
-        v |= 1U << bit;
-      } else if (r < abc) {
-        u |= 1U << bit;
-      } else {
-        u |= 1U << bit;
-        v |= 1U << bit;
-      }
+  const unsigned T = Threads();
+  std::vector<uint64_t> uniq;
+  uint64_t next_chunk = 0, drawn = 0;
+  double rate = 1.0;  // distinct edges gained per draw in the last batch
+  while (uniq.size() < target_edges) {
+    const double need = static_cast<double>(target_edges - uniq.size()) / std::max(rate, 0.05);
+    const uint64_t nchunks = std::max<uint64_t>(1, static_cast<uint64_t>(std::ceil(need / kChunk)));
+    std::vector<uint64_t> batch(nchunks * kChunk);
+    std::vector<uint64_t> filled(nchunks, 0);
+    std::vector<std::thread> ts;
+    for (unsigned t = 0; t < T; ++t) {
+      ts.emplace_back([&, t] {
+        for (uint64_t k = t; k < nchunks; k += T) {
+          Rng rng(seed ^ (0x9e3779b97f4a7c15ULL * (next_chunk + k + 1)));
+          uint64_t* out = batch.data() + k * kChunk;
+          uint64_t f = 0;
+          for (uint64_t d = 0; d < kChunk; ++d) {
+            uint32_t u = 0, v = 0;
+            for (int bit = scale - 1; bit >= 0; --bit) {
+              const double r = rng.Uniform();
+              if (r < a) {
+              } else if (r < ab) {
+                v |= 1U << bit;
+              } else if (r < abc) {
+                u |= 1U << bit;
+              } else {
+                u |= 1U << bit;
+                v |= 1U << bit;
+              }
+            }
+            if (u == v) continue;
+            out[f++] = (static_cast<uint64_t>(std::min(u, v)) << 32) | std::max(u, v);
+          }
+          filled[k] = f;
+        }
+      });
     }
-    if (u == v) continue;
-    set.Insert(u, v);
+    for (auto& th : ts) th.join();
+    uint64_t w = 0;
+    for (uint64_t k = 0; k < nchunks; ++k) {
+      std::memmove(batch.data() + w, batch.data() + k * kChunk, filled[k] * sizeof(uint64_t));
+      w += filled[k];
+    }
+    batch.resize(w);
+    ParallelSort(batch);
+    batch.erase(std::unique(batch.begin(), batch.end()), batch.end());
+    const size_t before = uniq.size();
+    std::vector<uint64_t> merged(uniq.size() + batch.size());
+    merged.erase(std::set_union(uniq.begin(), uniq.end(), batch.begin(), batch.end(), merged.begin()), merged.end());
+    uniq.swap(merged);
+    next_chunk += nchunks;
+    drawn += nchunks * kChunk;
+    rate = static_cast<double>(uniq.size() - before) / static_cast<double>(nchunks * kChunk);
+    std::fprintf(stderr, "rmat: %llu draws, %zu distinct edges\n", static_cast<unsigned long long>(drawn),
+                 uniq.size());
   }
-  std::vector<uint64_t> keys = set.Sorted();
   std::vector<uint32_t> relabel(static_cast<size_t>(1) << scale, 0);
-  for (uint64_t k : keys) {
+  for (uint64_t k : uniq) {
     relabel[k >> 32] = 1;
     relabel[k & 0xffffffffU] = 1;
   }
   uint32_t next = 0;
   for (auto& r : relabel) r = r ? next++ : ~0U;
-  for (auto& k : keys) {
+  for (auto& k : uniq) {
     const uint64_t u = relabel[k >> 32], v = relabel[k & 0xffffffffU];
     k = (std::min(u, v) << 32) | std::max(u, v);
   }
-  std::sort(keys.begin(), keys.end());
-  return keys;
+  ParallelSort(uniq);
+  std::fprintf(stderr, "rmat: %u nodes after dropping isolated ids\n", next);
+  return uniq;
 }
 
 }  // namespace
@@ -391,7 +477,12 @@ int main(int argc, char** argv) {
     } else {
       throw std::runtime_error("unknown workload " + which);
     }
-    WriteEdges(edges, argv[2]);
+    const std::string out = argv[2];
+    if (out.size() > 4 && out.compare(out.size() - 4, 4, ".bin") == 0) {
+      WriteEdgesBinary(edges, argv[2]);
+    } else {
+      WriteEdges(edges, argv[2]);
+    }
     std::fprintf(stderr, "%s: %zu edges -> %s\n", which.c_str(), edges.size(), argv[2]);
   } catch (const std::exception& e) {
     std::fprintf(stderr, "Error: %s\n", e.what());

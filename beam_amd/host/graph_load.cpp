@@ -63,6 +63,31 @@ Graph LoadEdgeList(const std::string& path) {
   long long max_node = -1, line_no = 0;
   const char* p = buf.data();
   const char* end = p + buf.size();
+  if (buf.size() >= 16 && std::memcmp(buf.data(), "BEAMEDG1", 8) == 0) {
+    // Binary edge list (not a reference format): "BEAMEDG1", uint64 edge
+    // count, uint32 (u, v) pairs; edge k is reported as line k + 1.
+    uint64_t m = 0;
+    std::memcpy(&m, buf.data() + 8, sizeof(m));
+    if (m > (buf.size() - 16) / 8 || buf.size() != 16 + 8 * m) {
+      throw std::runtime_error("Malformed binary edge list: size does not match the edge count.");
+    }
+    us.resize(m);
+    vs.resize(m);
+    for (uint64_t k = 0; k < m; ++k) {
+      uint32_t e[2];
+      std::memcpy(e, buf.data() + 16 + 8 * k, sizeof(e));
+      const long long u = e[0], v = e[1];
+      if (u > std::numeric_limits<int>::max() || v > std::numeric_limits<int>::max()) {
+        throw std::runtime_error("Node id exceeds int range at line " + std::to_string(k + 1) + ".");
+      }
+      if (u == v) throw std::runtime_error("Self-loop is not supported at line " + std::to_string(k + 1) + ".");
+      us[k] = static_cast<int>(u);
+      vs[k] = static_cast<int>(v);
+      max_node = std::max({max_node, u, v});
+    }
+    std::vector<char>().swap(buf);
+    p = end = nullptr;
+  }
   while (p < end) {
     const char* eol = static_cast<const char*>(std::memchr(p, '\n', static_cast<size_t>(end - p)));
     if (!eol) eol = end;
