@@ -114,6 +114,7 @@ def library() -> ctypes.CDLL:
         "beam_hip_commit": (ctypes.c_int, [P, P, P, i32]),
         "beam_hip_reset": (ctypes.c_int, [P]),
         "beam_hip_profile": (ctypes.c_int, [P, i32]),
+        "beam_hip_set_split": (ctypes.c_int, [P, i32]),
         "beam_hip_kernel_profile": (ctypes.c_int, [P, P]),
         "beam_hip_partition_hash": (ctypes.c_uint64, [P, i32]),
         "beam_hip_trace": (ctypes.c_int, [P, i32]),
@@ -147,7 +148,7 @@ def exported_symbols() -> list[str]:
         "beam_hip_destroy", "beam_hip_run", "beam_hip_iterate", "beam_hip_labels", "beam_hip_supernodes", "beam_hip_stats",
         "beam_hip_round_stats", "beam_hip_timing", "beam_hip_encoding_counts", "beam_hip_upload_state",
         "beam_hip_nnz", "beam_hip_download_state", "beam_hip_score_pairs", "beam_hip_merge_groups",
-        "beam_hip_divide", "beam_hip_commit", "beam_hip_reset", "beam_hip_profile",
+        "beam_hip_divide", "beam_hip_commit", "beam_hip_reset", "beam_hip_profile", "beam_hip_set_split",
         "beam_hip_kernel_profile", "beam_hip_partition_hash", "beam_hip_trace", "beam_hip_trace_fetch",
         "beam_hip_ranker_last_error", "beam_hip_ranker_forward", "beam_hip_ranker_gradient",
         "beam_hip_ranker_adam", "beam_hip_ranker_train", "beam_hip_ranker_stats", "beam_hip_ranker_get",
@@ -229,6 +230,10 @@ class Engine:
 
     def set_profile(self, enable: bool) -> None:
         _check(self.lib.beam_hip_profile(self.h, 1 if enable else 0))
+
+    def set_split(self, min_members: int) -> None:
+        """Groups of at least min_members members are split over workgroups (0: never)."""
+        _check(self.lib.beam_hip_set_split(self.h, min_members))
 
     def kernel_profile(self) -> dict:
         out = np.zeros(8, dtype=np.float64)

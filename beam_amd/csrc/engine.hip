@@ -516,6 +516,7 @@ struct Engine {
   // Host-side bookkeeping
   int64_t divide_max_group = 0;
   int64_t last_max_group = 0;     // largest group of the current iteration
+  int split_min = kSplitG;        // groups of at least this many members are split (0: never)
   int64_t divide_fallback_splits = 0;
   std::vector<int64_t> round_rows;  // 10 per iteration
   std::vector<Counters> iter_ctr;
@@ -954,11 +955,12 @@ void Engine::LaunchRound(int G, int nblocks, int block_size, double theta, Count
   mp.trace = (trace_buf != nullptr && tracing_now) ? trace_buf : nullptr;
   mp.round = cur;
   mp.todo = todo[epoch & 1];
-  const bool split_now = last_max_group >= kSplitG;
+  const bool split_now = split_min > 0 && last_max_group >= split_min;
   mp.split = split_now ? split : nullptr;
   mp.pair_items = pair_items;
   mp.row_items = row_items;
   mp.split_cap = kSplitCap;
+  mp.split_min = split_min;
   mp.item_cap = kSplitItems;
   SetRankerParams(mp);
   if (profile) {
@@ -1520,6 +1522,14 @@ int beam_hip_reset(beam_hip_engine* engine) {
 int beam_hip_profile(beam_hip_engine* engine, int32_t enable) {
   engine->e.profile = enable != 0;
   return 0;
+}
+
+int beam_hip_set_split(beam_hip_engine* engine, int32_t min_members) {
+  return Guard([&] {
+    if (min_members < 0) throw std::invalid_argument("split threshold must be >= 0");
+    engine->e.split_min = min_members;
+    return 0;
+  });
 }
 
 int beam_hip_kernel_profile(beam_hip_engine* engine, double* out) {

@@ -137,6 +137,7 @@ struct MergeParams {
   int2* pair_items;         // (record, chunk)
   int2* row_items;          // (record, chunk); record < 0: unused
   int split_cap;            // records
+  int split_min;            // groups of at least this many members are split
   int item_cap;             // entries of each item list
   // Learned ranker (kRankOff: plain exact scoring of every candidate).
   int ranker_mode;

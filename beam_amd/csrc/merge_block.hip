@@ -555,7 +555,8 @@ constexpr uint32_t kPairChunk = 4096;  // bucket pairs per pair-kernel item
 // Candidate generation, exact scoring and selection for group gidx; results
 // go to sel_out (pairs), sel_count[slot], slot_stats[slot].  Every early exit
 // is taken by the whole workgroup.  In round mode with split records, a group
-// of at least kSplitG members becomes a front (its record is rec).
+// of at least p.split_min members (and more than kSmallG) becomes a front (its
+// record is rec).
 template <bool kRanker>
 __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int slot, int32_t* sel_out, int item,
                                            unsigned char* s_region, int mode, SplitRec* rec) {
@@ -576,7 +577,7 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
     return;
   }
   __shared__ int s_mode, s_rslot, s_pb;
-  if (mode == kModeFull && p.split != nullptr && g >= kSplitG) {
+  if (mode == kModeFull && p.split != nullptr && g >= max(p.split_min, kSmallG + 1)) {
     // Reserve a record and the row chunks; without room the group stays on
     // this workgroup (reserved row entries are then marked unused).
     if (tid == 0) {

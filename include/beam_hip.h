@@ -141,6 +141,12 @@ int beam_hip_reset(beam_hip_engine* engine);
  * kernel (candidate generation + exact scoring + selection). */
 int beam_hip_profile(beam_hip_engine* engine, int32_t enable);
 
+/* Groups of at least min_members members are split over the pair and row
+ * kernels (default 128; 0 never splits).  No reference counterpart: a
+ * tuning knob whose every value gives the same partition, counters and
+ * output, which the tests check. */
+int beam_hip_set_split(beam_hip_engine* engine, int32_t min_members);
+
 /* Since the last reset: out[0] = summed group-kernel time in ms (profiling
  * on), out[1] = timed launches, out[2] = member rows read, out[3] = their
  * cross entries, out[4] = candidate pairs scored, out[5] = merge rounds

@@ -266,3 +266,27 @@ def test_end_to_end_matches_reference(graph_path, name, seed):
     ).encode()
     assert text.count(b"\n") == want["lines"]
     assert hashlib.sha256(text).hexdigest() == want["sha256"]
+
+
+@pytest.mark.parametrize("name,seed", [("am", 2), ("yo", 3)])
+def test_split_threshold_does_not_change_results(graph_path, name, seed):
+    """Splitting groups over the pair and row kernels is a scheduling choice:
+    never splitting, splitting every group above 64 members and the default
+    all give the reference's partition, counters and per-iteration rows."""
+    print(f"case: split thresholds {name}", flush=True)
+    gold = _golden(name)
+    m = gold["metrics"]
+    graph = beam_amd.load_graph(graph_path(name))
+    engine = beam_amd.Engine(graph, seed=seed)
+    for threshold in (0, 65, 128):
+        engine.set_split(threshold)
+        engine.reset()
+        engine.run(20)
+        assert beam_amd.partition_hash(engine.labels()) == int(m["partition_hash"]), threshold
+        enc = engine.encoding_counts()
+        for k in ("P", "Cp", "Cm"):
+            assert enc[k] == int(m[k]), (threshold, k)
+        stats = engine.stats()
+        for k in GOLDEN_KEYS:
+            assert stats[k] == int(m[k]), (threshold, k)
+        assert engine.rounds() == gold["rounds"], threshold
