@@ -960,25 +960,37 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
       if (inline_pairs) BucketPairs(A, blk, nzp, m_size, m_rep, g, PT, NB, shift, nblk, 0, PT, p.err);
     }
     stamp(15);
-    // Phase 6: direct edges between members (sweg.cpp:1905-1924).
-    for (uint32_t e = tid; e < E; e += kThreads) {
-      if (A.ent_slot[e] < 0) continue;
-      const int t = A.ent_t[e];
-      uint32_t h = HashInt(static_cast<uint32_t>(t)) & (kMemberSlots - 1);
-      int j = -1;
-      while (true) {
-        const int key = mh_key[h];
-        if (key == t) {
-          j = mh_val[h];
-          break;
-        }
-        if (key == -1) break;
-        h = (h + 1) & (kMemberSlots - 1);
+    // Phase 6: direct edges between members (sweg.cpp:1905-1924); four
+    // entries per thread at a time, their loads in flight together.
+    for (uint32_t e0 = tid; e0 < E; e0 += 4 * kThreads) {
+      int t4[4], i4[4], sl4[4], c4[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t e = min(e0 + u * kThreads, E - 1);
+        sl4[u] = e0 + u * kThreads < E ? A.ent_slot[e] : -1;
+        t4[u] = A.ent_t[e];
+        i4[u] = small ? s_ent_i[e] : A.g_ent_i[e];
+        c4[u] = A.ent_e[e];
       }
-      const int i = small ? s_ent_i[e] : A.g_ent_i[e];
-      if (j > i) {
-        atomicAdd(reinterpret_cast<unsigned long long*>(&A.proxy[i * g + j]),
-                  static_cast<unsigned long long>(A.ent_e[e]));
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (sl4[u] < 0) continue;
+        const int t = t4[u];
+        uint32_t h = HashInt(static_cast<uint32_t>(t)) & (kMemberSlots - 1);
+        int j = -1;
+        while (true) {
+          const int key = mh_key[h];
+          if (key == t) {
+            j = mh_val[h];
+            break;
+          }
+          if (key == -1) break;
+          h = (h + 1) & (kMemberSlots - 1);
+        }
+        if (j > i4[u]) {
+          atomicAdd(reinterpret_cast<unsigned long long*>(&A.proxy[i4[u] * g + j]),
+                    static_cast<unsigned long long>(c4[u]));
+        }
       }
     }
     __syncthreads();
