@@ -92,6 +92,13 @@ __device__ __forceinline__ unsigned long long Uniform(unsigned long long x) {
   return (static_cast<unsigned long long>(hi) << 32) | lo;
 }
 
+// Lane `lane`'s value of x (a wave-uniform result).
+__device__ __forceinline__ unsigned long long ReadLane64(unsigned long long x, int lane) {
+  const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(x)), lane));
+  const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(x >> 32)), lane));
+  return (static_cast<unsigned long long>(hi) << 32) | lo;
+}
+
 __device__ __forceinline__ int LaneId() { return static_cast<int>(threadIdx.x & (kWave - 1)); }
 __device__ __forceinline__ int WaveId() { return static_cast<int>(threadIdx.x / kWave); }
 

@@ -62,8 +62,9 @@ struct GroupArrays {
   int32_t* ent_st;      // target size
   int32_t* ent_slot;    // target table slot, -1 for the self entry
   int32_t* th_key;
-  uint32_t* th_cnt;     // counts, then exclusive offsets (TC + 1)
-  uint32_t* pair_pref;  // pairs per slot, then exclusive offsets (TC + 1)
+  uint32_t* th_cnt;     // counts, then small groups: exclusive offsets (TC + 1);
+                        // large groups: the bucket's entry offset (kNoBucket: one entry)
+  uint32_t* pair_pref;  // small groups: pairs per slot, then exclusive offsets (TC + 1)
   long long* proxy;     // g x g, upper triangle
   uint32_t* candbits;   // g x g bits
   uint16_t* cand_a;
@@ -80,10 +81,9 @@ struct GroupArrays {
   int32_t* g_ent_rank;
   int32_t* g_bucket;    // large: (member << 32 | count) per bucket position, 8 B
   // Large groups' bucket-pair index: target size per slot, the slots with
-  // at least two entries compacted (position, {offset, size, target,
-  // target size}, pair prefix).
+  // at least two entries numbered ({offset, size, target, target size},
+  // pair prefix).
   int32_t* th_st;
-  uint32_t* nz_pos;
   uint4* nz_info;
   uint32_t* nz_pref;
   // Large groups: the bucket-pair block index and the members' approximate
@@ -157,7 +157,7 @@ __device__ __forceinline__ long long CorrOf(long long w) { return w >> 32; }
 // Scratch bytes of a large group and the layout of its arrays.
 __device__ __forceinline__ unsigned long long LargeBytes(uint32_t E, uint32_t TC, int g, uint32_t cmax) {
   const unsigned long long e64 = E, tc64 = TC, g64 = static_cast<unsigned long long>(g), c64 = cmax;
-  return e64 * 4 * 8 + tc64 * 4 + (tc64 + 1) * 8 + tc64 * 4 + (tc64 + 1) * 4 + 16 + tc64 * 16 + (tc64 + 1) * 4 + 16 +
+  return e64 * 4 * 8 + tc64 * 4 + (tc64 + 1) * 4 + tc64 * 4 + 16 + tc64 * 16 + (tc64 + 1) * 4 + 16 +
          g64 * g64 * 8 + (g64 * g64 / 32 + 1) * 4 + 16 + c64 * (8 + 8 + 2 + 2) + (c64 + 1) * 4 + g64 * 64 + 16 +
          e64 * 16 + 64 + (kBlkMaxBytes + 16) + g64 * 16 + 16;
 }
@@ -176,9 +176,8 @@ __device__ __forceinline__ void LargeLayout(unsigned char* r, uint32_t E, uint32
   A.g_bucket = reinterpret_cast<int32_t*>(r); r += E * 8ULL;  // 8-aligned: 24 E bytes precede it
   A.th_key = reinterpret_cast<int32_t*>(r); r += TC * 4ULL;
   A.th_cnt = reinterpret_cast<uint32_t*>(r); r += (TC + 1) * 4ULL;
-  A.pair_pref = reinterpret_cast<uint32_t*>(r); r += (TC + 1) * 4ULL;
+  A.pair_pref = nullptr;  // small groups only
   A.th_st = reinterpret_cast<int32_t*>(r); r += TC * 4ULL;
-  A.nz_pos = reinterpret_cast<uint32_t*>(r); r += (TC + 1) * 4ULL;
   r = reinterpret_cast<unsigned char*>((reinterpret_cast<uintptr_t>(r) + 15) & ~uintptr_t(15));
   A.nz_info = reinterpret_cast<uint4*>(r); r += TC * 16ULL;
   A.nz_pref = reinterpret_cast<uint32_t*>(r); r += (TC + 1) * 4ULL;
@@ -634,7 +633,8 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
   __shared__ uint32_t s_prefix, s_need;
   __shared__ int s_pruned;
   __shared__ unsigned long long s_best;
-  __shared__ unsigned long long s_pt64;
+  __shared__ unsigned long long s_pt64, s_pack;
+  __shared__ uint32_t s_ents;
   __shared__ int16_t s_topj[kWaves][kRankMax];
 
   // Phase 1: members, member lookup table, staged entry offsets.
@@ -691,7 +691,7 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
     A.cand_a = reinterpret_cast<uint16_t*>(c); c += kSmallC * 2;
     A.cand_b = reinterpret_cast<uint16_t*>(c);
     A.g_ent_i = A.g_ent_rank = A.g_bucket = A.th_st = A.mrep = A.msize = nullptr;
-    A.nz_pos = A.nz_pref = A.blk = nullptr;
+    A.nz_pref = A.blk = nullptr;
     A.nz_info = nullptr;
     A.maoc = nullptr;
     A.sig = s_sig;
@@ -844,120 +844,179 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
     __syncthreads();
 
     stamp(7);
-    // Phase 4: bucket offsets, scatter, pairs per bucket.
-    if (tid == 0) s_pt64 = 0;
-    ScanInPlace(A.th_cnt, static_cast<int>(TC), wave_tot);
+    if (small) {
+      // Phase 4: bucket offsets, scatter, pairs per bucket.
+      ScanInPlace(A.th_cnt, static_cast<int>(TC), wave_tot);
 #pragma unroll 4
-    for (uint32_t e = tid; e < E; e += kThreads) {
-      const int sl = A.ent_slot[e];
-      if (sl < 0) continue;
-      const int rank = small ? s_ent_rank[e] : A.g_ent_rank[e];
-      const uint32_t pos = A.th_cnt[sl] + static_cast<uint32_t>(rank);
-      if (small) {
-        s_bucket[pos] = static_cast<int16_t>(e);
-      } else {
-        reinterpret_cast<unsigned long long*>(A.g_bucket)[pos] =
-            (static_cast<unsigned long long>(static_cast<uint32_t>(A.g_ent_i[e])) << 32) |
-            static_cast<uint32_t>(A.ent_e[e]);
+      for (uint32_t e = tid; e < E; e += kThreads) {
+        const int sl = A.ent_slot[e];
+        if (sl < 0) continue;
+        s_bucket[A.th_cnt[sl] + static_cast<uint32_t>(s_ent_rank[e])] = static_cast<int16_t>(e);
       }
-    }
-    unsigned long long pairs64 = 0;  // bucket sizes are at most g, so each count fits 32 bits
 #pragma unroll 8
-    for (uint32_t k = tid; k < TC; k += kThreads) {
-      const uint32_t s = A.th_cnt[k + 1] - A.th_cnt[k];
-      A.pair_pref[k] = s >= 2 ? s * (s - 1) / 2 : 0;
-      pairs64 += s >= 2 ? s * (s - 1) / 2 : 0;
-    }
-    if (!small) atomicAdd(&s_pt64, pairs64);
-    __syncthreads();
-    if (!small && s_pt64 > 0xffffffffULL) {  // the 32-bit pair indexing would wrap
-      if (tid == 0) atomicOr(p.err, kErrPairs);
-      return;
-    }
-    PT = Uniform(ScanInPlace(A.pair_pref, static_cast<int>(TC), wave_tot));
-
-    // Phase 5: bucket pair expansion (sweg.cpp:1863-1903): for two members
-    // sharing target t, delta = EC(a,t,ea) + EC(b,t,eb) - EC(a+b,t,ea+eb),
-    // accumulated only when positive.
-    // Small groups here, from LDS; large groups below.
-    for (uint32_t pi = tid; small && pi < PT; pi += kThreads) {
-      const int sl = UpperSegment(A.pair_pref, static_cast<int>(TC), pi);
-      const uint32_t l = pi - A.pair_pref[sl];
-      const uint32_t off = A.th_cnt[sl];
-      const uint32_t s = A.th_cnt[sl + 1] - off;
-      uint32_t x, y;
-      TriDecode(l, s, &x, &y);
-      const int ex = s_bucket[off + x];
-      const int ey = s_bucket[off + y];
-      const int i = s_ent_i[ex];
-      const int j = s_ent_i[ey];
-      const int t = A.ent_t[ex];
-      const int64_t st = A.ent_st[ex];
-      const int64_t ea = A.ent_e[ex], eb = A.ent_e[ey];
-      const int64_t sa = m_size[i], sb = m_size[j];
-      const int ra = m_rep[i], rb = m_rep[j];
-      const int64_t old_cost = EncodeCost(ra, sa, t, st, ea) + EncodeCost(rb, sb, t, st, eb);
-      const int64_t new_cost = EncodeCost(ra, sa + sb, t, st, ea + eb);
-      const int64_t delta = old_cost - new_cost;
-      // t is a common neighbour of members i and j: its term of their exact
-      // gain's common-neighbour correction goes into the high word.
-      const int64_t sw = sa + sb;
-      const int64_t corr = BlockCost(sw, st, ea + eb, false, p.err) - BlockCost(sw, st, ea, false, p.err) -
-                           BlockCost(sw, st, eb, false, p.err);
-      const unsigned long long add =
-          (delta > 0 ? static_cast<unsigned long long>(delta) : 0ULL) + (static_cast<unsigned long long>(corr) << 32);
-      if (add != 0) {
-        const int lo = min(i, j), hi = max(i, j);
-        atomicAdd(reinterpret_cast<unsigned long long*>(&A.proxy[lo * g + hi]), add);
-      }
-    }
-    if (!small && PT > 0) {
-      // Large groups: compact the buckets with pairs and index blocks of
-      // 2^shift pairs by their first bucket (in LDS on this workgroup, in the
-      // scratch for a front, whose pairs the pair kernel expands).
-#pragma unroll 8
-      for (uint32_t k = tid; k < TC; k += kThreads) A.nz_pos[k] = A.th_cnt[k + 1] - A.th_cnt[k] >= 2 ? 1U : 0U;
-      __syncthreads();
-      NB = Uniform(ScanInPlace(A.nz_pos, static_cast<int>(TC), wave_tot));
-      uint32_t* nzp = mode == kModeFull && NB <= static_cast<uint32_t>(kNzLds)
-                          ? reinterpret_cast<uint32_t*>(s_region + kNzPrefOffset)
-                          : A.nz_pref;
-#pragma unroll 4
       for (uint32_t k = tid; k < TC; k += kThreads) {
-        const uint32_t off = A.th_cnt[k], sz = A.th_cnt[k + 1] - off;
-        if (sz < 2) continue;
-        const uint32_t q = A.nz_pos[k];
-        A.nz_info[q] = make_uint4(off, sz, static_cast<uint32_t>(A.th_key[k]), static_cast<uint32_t>(A.th_st[k]));
-        nzp[q] = A.pair_pref[k];
-      }
-      if (tid == 0) nzp[NB] = PT;
-      __syncthreads();
-      while (((PT - 1) >> shift) + 1 > static_cast<uint32_t>(kBlkMax)) ++shift;
-      nblk = ((PT - 1) >> shift) + 1;
-      uint32_t* blk = mode == kModeFull ? reinterpret_cast<uint32_t*>(s_region) : A.blk;
-      for (uint32_t k = tid; k < NB; k += kThreads) {
-        const uint32_t pb = nzp[k], pe = nzp[k + 1];
-        for (uint32_t b = (pb + (1U << shift) - 1) >> shift; b <= ((pe - 1) >> shift); ++b) blk[b] = k;
+        const uint32_t n = A.th_cnt[k + 1] - A.th_cnt[k];
+        A.pair_pref[k] = n >= 2 ? n * (n - 1) / 2 : 0;
       }
       __syncthreads();
-      bool inline_pairs = mode == kModeFull;
-      if (mode == kModeFront) {
-        // Queue the pair chunks; without room they run here.
-        const int pc = static_cast<int>((PT + kPairChunk - 1) / kPairChunk);
+      PT = Uniform(ScanInPlace(A.pair_pref, static_cast<int>(TC), wave_tot));
+
+      // Phase 5: bucket pair expansion (sweg.cpp:1863-1903): for two members
+      // sharing target t, delta = EC(a,t,ea) + EC(b,t,eb) - EC(a+b,t,ea+eb),
+      // accumulated only when positive.  Large groups: BucketPairs.
+      for (uint32_t pi = tid; pi < PT; pi += kThreads) {
+        const int sl = UpperSegment(A.pair_pref, static_cast<int>(TC), pi);
+        const uint32_t l = pi - A.pair_pref[sl];
+        const uint32_t off = A.th_cnt[sl];
+        const uint32_t n = A.th_cnt[sl + 1] - off;
+        uint32_t x, y;
+        TriDecode(l, n, &x, &y);
+        const int ex = s_bucket[off + x];
+        const int ey = s_bucket[off + y];
+        const int i = s_ent_i[ex];
+        const int j = s_ent_i[ey];
+        const int t = A.ent_t[ex];
+        const int64_t st = A.ent_st[ex];
+        const int64_t ea = A.ent_e[ex], eb = A.ent_e[ey];
+        const int64_t sa = m_size[i], sb = m_size[j];
+        const int ra = m_rep[i], rb = m_rep[j];
+        const int64_t old_cost = EncodeCost(ra, sa, t, st, ea) + EncodeCost(rb, sb, t, st, eb);
+        const int64_t new_cost = EncodeCost(ra, sa + sb, t, st, ea + eb);
+        const int64_t delta = old_cost - new_cost;
+        // t is a common neighbour of members i and j: its term of their exact
+        // gain's common-neighbour correction goes into the high word.
+        const int64_t sw = sa + sb;
+        const int64_t corr = BlockCost(sw, st, ea + eb, false, p.err) - BlockCost(sw, st, ea, false, p.err) -
+                             BlockCost(sw, st, eb, false, p.err);
+        const unsigned long long add =
+            (delta > 0 ? static_cast<unsigned long long>(delta) : 0ULL) + (static_cast<unsigned long long>(corr) << 32);
+        if (add != 0) {
+          const int lo = min(i, j), hi = max(i, j);
+          atomicAdd(reinterpret_cast<unsigned long long*>(&A.proxy[lo * g + hi]), add);
+        }
+      }
+    } else {
+      // Phase 4, large groups: one pass over the slots numbers the buckets
+      // with at least two entries (singleton buckets never pair).  Each wave
+      // takes 64 slots at a time, scans their pair and entry counts across
+      // its lanes and reserves bucket numbers, pairs and entry space with one
+      // LDS atomic; bucket number and first pair share one 64-bit counter
+      // (number << 34 | pairs), so the pair prefix ascends with the bucket
+      // number.  A bucket's slot count is replaced by its entry offset
+      // (kNoBucket for singletons).
+      constexpr uint32_t kNoBucket = 0xffffffffU;
+      if (tid == 0) {
+        s_pack = 0;
+        s_ents = 0;
+        s_pt64 = 0;
+      }
+      __syncthreads();
+      {
+        const int lane = LaneId();
+        const unsigned long long below = (1ULL << lane) - 1ULL;
+        for (uint32_t c0 = static_cast<uint32_t>(WaveId()) * kWave; c0 < TC; c0 += 4 * kThreads) {
+          uint32_t cv[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const uint32_t k = c0 + u * kThreads + lane;
+            cv[u] = k < TC ? A.th_cnt[k] : 0;
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const uint32_t k = c0 + u * kThreads + lane;
+            const uint32_t n = cv[u];
+            const bool nz = n >= 2;
+            const unsigned long long m = __ballot(nz);
+            if (n == 1) A.th_cnt[k] = kNoBucket;
+            if (m == 0) continue;
+            const uint32_t pr = nz ? n * (n - 1) / 2 : 0, en = nz ? n : 0;
+            const uint32_t pr_inc = WaveInclusiveScan(pr), en_inc = WaveInclusiveScan(en);
+            unsigned long long old = 0;
+            uint32_t eold = 0;
+            if (lane == kWave - 1) {
+              old = atomicAdd(&s_pack, (static_cast<unsigned long long>(__popcll(m)) << 34) + pr_inc);
+              eold = atomicAdd(&s_ents, en_inc);
+              atomicAdd(&s_pt64, static_cast<unsigned long long>(pr_inc));
+            }
+            old = ReadLane64(old, kWave - 1);
+            eold = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(eold), kWave - 1));
+            if (!nz) continue;
+            const uint32_t q = static_cast<uint32_t>(old >> 34) + static_cast<uint32_t>(__popcll(m & below));
+            const uint32_t eoff = eold + en_inc - en;
+            if (q < TC) {  // q wraps only when the pairs overflow, which the check below rejects
+              A.nz_info[q] = make_uint4(eoff, n, static_cast<uint32_t>(A.th_key[k]), static_cast<uint32_t>(A.th_st[k]));
+              A.nz_pref[q] = static_cast<uint32_t>(old) + pr_inc - pr;
+            }
+            A.th_cnt[k] = eoff;
+          }
+        }
+      }
+      __syncthreads();
+      if (s_pt64 > 0xffffffffULL) {  // the 32-bit pair indexing would wrap
+        if (tid == 0) atomicOr(p.err, kErrPairs);
+        return;
+      }
+      NB = Uniform(static_cast<uint32_t>(s_pack >> 34));
+      PT = Uniform(static_cast<uint32_t>(s_pt64));
+      // Scatter: member << 32 | count at the bucket's offset + rank; four
+      // entries per thread at a time.
+      for (uint32_t e0 = tid; e0 < E; e0 += 4 * kThreads) {
+        int sl4[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) sl4[u] = e0 + u * kThreads < E ? A.ent_slot[e0 + u * kThreads] : -1;
+        uint32_t off4[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) off4[u] = sl4[u] >= 0 ? A.th_cnt[sl4[u]] : kNoBucket;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          if (off4[u] == kNoBucket) continue;
+          const uint32_t e = e0 + u * kThreads;
+          reinterpret_cast<unsigned long long*>(A.g_bucket)[off4[u] + static_cast<uint32_t>(A.g_ent_rank[e])] =
+              (static_cast<unsigned long long>(static_cast<uint32_t>(A.g_ent_i[e])) << 32) |
+              static_cast<uint32_t>(A.ent_e[e]);
+        }
+      }
+      if (PT > 0) {
+        // Blocks of 2^shift pairs indexed by their first bucket (in LDS on
+        // this workgroup, with the pair prefix when it fits; in the scratch
+        // for a front, whose pairs the pair kernel expands).
+        uint32_t* nzp = A.nz_pref;
+        if (mode == kModeFull && NB <= static_cast<uint32_t>(kNzLds)) {
+          nzp = reinterpret_cast<uint32_t*>(s_region + kNzPrefOffset);
+          for (uint32_t k = tid; k < NB; k += kThreads) nzp[k] = A.nz_pref[k];
+        }
         if (tid == 0) {
-          const int pb = atomicAdd(&p.round->npair_items, pc);
-          s_flag = pb + pc <= p.item_cap ? 1 : 0;
-          s_pb = pb;
+          nzp[NB] = PT;
+          if (nzp != A.nz_pref) A.nz_pref[NB] = PT;
         }
         __syncthreads();
-        const int pb = s_pb;
-        const int ok = s_flag;
+        while (((PT - 1) >> shift) + 1 > static_cast<uint32_t>(kBlkMax)) ++shift;
+        nblk = ((PT - 1) >> shift) + 1;
+        uint32_t* blk = mode == kModeFull ? reinterpret_cast<uint32_t*>(s_region) : A.blk;
+        for (uint32_t k = tid; k < NB; k += kThreads) {
+          const uint32_t pb = nzp[k], pe = nzp[k + 1];
+          for (uint32_t b = (pb + (1U << shift) - 1) >> shift; b <= ((pe - 1) >> shift); ++b) blk[b] = k;
+        }
         __syncthreads();
-        for (int c = tid; c < pc && pb + c < p.item_cap; c += kThreads)
-          p.pair_items[pb + c] = make_int2(ok ? static_cast<int>(rec - p.split) : -1, c);
-        inline_pairs = ok == 0;
+        bool inline_pairs = mode == kModeFull;
+        if (mode == kModeFront) {
+          // Queue the pair chunks; without room they run here.
+          const int pc = static_cast<int>((PT + kPairChunk - 1) / kPairChunk);
+          if (tid == 0) {
+            const int pb = atomicAdd(&p.round->npair_items, pc);
+            s_flag = pb + pc <= p.item_cap ? 1 : 0;
+            s_pb = pb;
+          }
+          __syncthreads();
+          const int pb = s_pb;
+          const int ok = s_flag;
+          __syncthreads();
+          for (int c = tid; c < pc && pb + c < p.item_cap; c += kThreads)
+            p.pair_items[pb + c] = make_int2(ok ? static_cast<int>(rec - p.split) : -1, c);
+          inline_pairs = ok == 0;
+        }
+        if (inline_pairs) BucketPairs(A, blk, nzp, m_size, m_rep, g, PT, NB, shift, nblk, 0, PT, p.err);
       }
-      if (inline_pairs) BucketPairs(A, blk, nzp, m_size, m_rep, g, PT, NB, shift, nblk, 0, PT, p.err);
     }
     stamp(15);
     // Phase 6: direct edges between members (sweg.cpp:1905-1924); four
