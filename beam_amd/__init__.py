@@ -232,7 +232,8 @@ class Engine:
         _check(self.lib.beam_hip_profile(self.h, 1 if enable else 0))
 
     def set_split(self, min_members: int) -> None:
-        """Groups of at least min_members members are split over workgroups (0: never)."""
+        """Groups of at least min_members members are split over workgroups (0: never;
+        default 65), in iterations whose largest group has max(min_members, 128)+ members."""
         _check(self.lib.beam_hip_set_split(self.h, min_members))
 
     def kernel_profile(self) -> dict:

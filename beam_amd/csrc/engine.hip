@@ -955,7 +955,7 @@ void Engine::LaunchRound(int G, int nblocks, int block_size, double theta, Count
   mp.trace = (trace_buf != nullptr && tracing_now) ? trace_buf : nullptr;
   mp.round = cur;
   mp.todo = todo[epoch & 1];
-  const bool split_now = split_min > 0 && last_max_group >= split_min;
+  const bool split_now = split_min > 0 && last_max_group >= std::max(split_min, kSplitGate);
   mp.split = split_now ? split : nullptr;
   mp.pair_items = pair_items;
   mp.row_items = row_items;
@@ -973,7 +973,7 @@ void Engine::LaunchRound(int G, int nblocks, int block_size, double theta, Count
     }
     HIP_CHECK(hipEventRecord(ev_pool[ev_used++], stream));
   }
-  // In iterations with groups of at least kSplitG members, the group kernel
+  // In iterations with split groups (see kSplitGate), the group kernel
   // stages those and the pair and row kernels finish them (both exit at
   // once in rounds without any).
   if (ranker_mode_now != kRankOff) {

@@ -70,7 +70,7 @@ struct RoundState {
   int j;       // blocks this round commits
   int ntodo;   // groups queued for scoring in this round
   int steal;   // scoring work counter
-  // Groups of at least kSplitG members are split over workgroups: the group
+  // Groups of kSplitG+ members are split over workgroups (see kSplitGate): the group
   // kernel stages them (front), the pair kernel expands their bucket pairs
   // in chunks, the top-k kernel ranks their rows in chunks and the
   // workgroup that finishes a group's last row chunk scores and selects it.
@@ -82,7 +82,12 @@ struct RoundState {
   int pad[3];
 };
 
-constexpr int kSplitG = 128;
+// Groups of at least kSplitG members (the default threshold) are split, in
+// iterations whose largest group has at least max(threshold, kSplitGate)
+// members: in iterations with only medium groups the two extra launches per
+// round cost more than they save.
+constexpr int kSplitG = 65;
+constexpr int kSplitGate = 128;
 
 // One split group of a round (written by its front, read by the kernels
 // after it).

@@ -271,8 +271,9 @@ def test_end_to_end_matches_reference(graph_path, name, seed):
 @pytest.mark.parametrize("name,seed", [("am", 2), ("yo", 3)])
 def test_split_threshold_does_not_change_results(graph_path, name, seed):
     """Splitting groups over the pair and row kernels is a scheduling choice:
-    never splitting, splitting every group above 64 members and the default
-    all give the reference's partition, counters and per-iteration rows."""
+    never splitting, splitting groups of 65+ members (the default) or only
+    those of 128+ all give the reference's partition, counters and
+    per-iteration rows."""
     print(f"case: split thresholds {name}", flush=True)
     gold = _golden(name)
     m = gold["metrics"]
