@@ -721,6 +721,13 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
   if (mode == kModeBack) {
     PT = rec->PT;
     for (int i = tid; i < g; i += kThreads) m_aoc[i] = A.maoc[i];
+    // The kept-pair bits the row kernel set, into the LDS region for the
+    // candidate listing (below the join tables, like a full group's).
+    uint32_t* bits = reinterpret_cast<uint32_t*>(s_region);
+    const uint32_t nw = (static_cast<uint32_t>(g) * static_cast<uint32_t>(g) + 31) / 32;
+#pragma unroll 4
+    for (uint32_t k = tid; k < nw; k += kThreads) bits[k] = A.candbits[k];
+    A.candbits = bits;
     __syncthreads();
   } else {
     // Clear the target table, the proxy matrix and the candidate bits.
@@ -1310,45 +1317,34 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
 
   stamp(9);
   // Phase 8: candidate pairs in ascending (a_idx, b_idx) order
-  // (sweg.cpp:1979-1989).
-  // One wave per row: lane l tests partner lo + 1 + 64 r + l; ballots count
-  // and place the kept partners in ascending order.
+  // (sweg.cpp:1979-1989).  A kept pair a < b is bit a * g + b of the bitmap,
+  // so that order is the order of the set bits: each thread counts the bits
+  // of a run of words, a block scan places the runs, and each thread writes
+  // its pairs in bit order.
+  uint32_t C;
   {
-    const int lane = LaneId();
-    for (int lo = WaveId(); lo < g; lo += kWaves) {
-      uint32_t cnt = 0;
-      for (int hb = lo + 1; hb < g; hb += kWave) {
-        const int hi = hb + lane;
-        const uint32_t bit = static_cast<uint32_t>(lo) * g + static_cast<uint32_t>(hi);
-        const bool kept = hi < g && ((A.candbits[bit >> 5] >> (bit & 31)) & 1U);
-        cnt += static_cast<uint32_t>(__popcll(__ballot(kept)));
+    const uint32_t nw = (static_cast<uint32_t>(g) * static_cast<uint32_t>(g) + 31) / 32;
+    const uint32_t per = (nw + kThreads - 1) / kThreads;
+    const uint32_t w0 = min(nw, static_cast<uint32_t>(tid) * per), w1 = min(nw, w0 + per);
+    uint32_t cnt = 0;
+    for (uint32_t k = w0; k < w1; ++k) cnt += static_cast<uint32_t>(__popc(A.candbits[k]));
+    uint32_t total = 0;
+    uint32_t pos = BlockExclusiveScan(cnt, wave_tot, &total);
+    for (uint32_t k = w0; k < w1; ++k) {
+      uint32_t w = A.candbits[k];
+      while (w != 0) {
+        const uint32_t bit = k * 32 + static_cast<uint32_t>(__ffs(w) - 1);
+        w &= w - 1;
+        const uint32_t lo = bit / static_cast<uint32_t>(g);
+        A.cand_a[pos] = static_cast<uint16_t>(lo);
+        A.cand_b[pos] = static_cast<uint16_t>(bit - lo * static_cast<uint32_t>(g));
+        ++pos;
       }
-      if (lane == 0) m_rowcnt[lo] = cnt;
     }
+    C = Uniform(total);
     __syncthreads();
   }
-  uint32_t C = Uniform(ScanInPlace(m_rowcnt, g, wave_tot));
   const uint32_t C0 = C;
-  {
-    const int lane = LaneId();
-    const unsigned long long below = (1ULL << lane) - 1ULL;
-    for (int lo = WaveId(); lo < g; lo += kWaves) {
-      uint32_t pos = m_rowcnt[lo];
-      for (int hb = lo + 1; hb < g; hb += kWave) {
-        const int hi = hb + lane;
-        const uint32_t bit = static_cast<uint32_t>(lo) * g + static_cast<uint32_t>(hi);
-        const bool kept = hi < g && ((A.candbits[bit >> 5] >> (bit & 31)) & 1U);
-        const unsigned long long mask = __ballot(kept);
-        if (kept) {
-          const uint32_t q = pos + static_cast<uint32_t>(__popcll(mask & below));
-          A.cand_a[q] = static_cast<uint16_t>(lo);
-          A.cand_b[q] = static_cast<uint16_t>(hi);
-        }
-        pos += static_cast<uint32_t>(__popcll(mask));
-      }
-    }
-    __syncthreads();
-  }
 
   // Learned ranker: features of a candidate pair from the staged group.
   const FeatureSource fsrc{m_size, m_len, m_int, m_aoc, A.proxy, A.sig, g, p.theta};
