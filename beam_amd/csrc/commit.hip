@@ -330,6 +330,7 @@ __global__ __launch_bounds__(kThreads) void TouchRound(RoundParams p) {
     p.next->ntodo = 0;
     p.next->steal = 0;
     p.next->nsplit = p.next->npair_items = p.next->nrow_items = p.next->steal_pairs = p.next->steal_rows = 0;
+    p.next->nscore_items = p.next->steal_score = 0;
     *p.union_top = 0;
     p.round_ctr[1] += static_cast<unsigned long long>(p.cur->ntodo);
   }

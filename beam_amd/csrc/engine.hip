@@ -258,6 +258,7 @@ __global__ void InitRound(RoundState* rs, int end, int32_t* naff, unsigned long 
   rs->ntodo = 0;
   rs->steal = 0;
   rs->nsplit = rs->npair_items = rs->nrow_items = rs->steal_pairs = rs->steal_rows = 0;
+  rs->nscore_items = rs->steal_score = 0;
   *naff = 0;
   *group_top = 0;
 }
@@ -489,6 +490,7 @@ struct Engine {
   SplitRec* split = nullptr;      // [kSplitCap] split groups of a round
   int2* pair_items = nullptr;     // [kSplitItems]
   int2* row_items = nullptr;      // [kSplitItems]
+  int2* score_items = nullptr;    // [kSplitItems]
   unsigned long long* round_ctr = nullptr;  // [2] productive rounds, groups scored
   int horizon_blocks = 64;
   int score_grid = 256;
@@ -577,7 +579,7 @@ Engine::~Engine() {
                   open_rep, mh,       recs[0],  recs[1],   flag,     excl,       rid,       run_start, run_open,
                   run_seg,  elem_open, new_idx, gflag,     gstart,   glen,       d_total,   d_maxlen, sel,
                   sel_count, mark_dest, dest,   mark_keep, mark_row, aff,        naff,      scratch_a, scratch_u,
-                  slot_stats, touch,  commit_out, node_group, gmark,     todo[0],   todo[1], rs, split, pair_items, row_items, round_ctr, samples, sample_target, sample_key,
+                  slot_stats, touch,  commit_out, node_group, gmark,     todo[0],   todo[1], rs, split, pair_items, row_items, score_items, round_ctr, samples, sample_target, sample_key,
                   sample_count, audit_out, audit_list, audit_rs,
                   tops,     ctr,      err,      enc,        trace_buf};
   for (void* p : ptrs)
@@ -665,6 +667,7 @@ void Engine::Create(const int64_t* h_row_ptr, const int32_t* h_col, int n_, long
   split = DevAlloc<SplitRec>(kSplitCap);
   pair_items = DevAlloc<int2>(kSplitItems);
   row_items = DevAlloc<int2>(kSplitItems);
+  score_items = DevAlloc<int2>(kSplitItems);
   round_ctr = DevAlloc<unsigned long long>(2);
   {
     int cus = 0, per_cu = 0;
@@ -959,6 +962,8 @@ void Engine::LaunchRound(int G, int nblocks, int block_size, double theta, Count
   mp.split = split_now ? split : nullptr;
   mp.pair_items = pair_items;
   mp.row_items = row_items;
+  // Plain method: split groups are scored in chunks over all workgroups.
+  mp.score_items = split_now && ranker_mode_now == kRankOff ? score_items : nullptr;
   mp.split_cap = kSplitCap;
   mp.split_min = split_min;
   mp.item_cap = kSplitItems;
@@ -987,6 +992,7 @@ void Engine::LaunchRound(int G, int nblocks, int block_size, double theta, Count
     if (split_now) {
       SplitPairsKernel<<<score_grid, 512, 0, stream>>>(mp);
       SplitRowsKernel<<<score_grid, 512, 0, stream>>>(mp);
+      SplitScoreKernel<<<score_grid, 512, 0, stream>>>(mp);
     }
   }
   if (profile) HIP_CHECK(hipEventRecord(ev_pool[ev_used++], stream));
@@ -1125,7 +1131,7 @@ void Engine::RankerSetupIteration(int iter, int G, double theta) {
   for (int k = 0; k < count; ++k) list[static_cast<size_t>(k)] = k * stride;
   HIP_CHECK(hipMemcpyAsync(audit_list, list.data(), sizeof(int32_t) * count, hipMemcpyHostToDevice, stream));
   HIP_CHECK(hipMemsetAsync(audit_out, 0, sizeof(long long) * 3 * count, stream));
-  const RoundState init{0, 0, 0, count, 0, 0, 0, 0, 0, 0, {0, 0, 0}};
+  const RoundState init{0, 0, 0, count, 0, 0, 0, 0, 0, 0, 0, 0, {0}};
   HIP_CHECK(hipMemcpyAsync(audit_rs, &init, sizeof(init), hipMemcpyHostToDevice, stream));
   HIP_CHECK(hipMemsetAsync(&tops[0], 0, sizeof(unsigned long long), stream));
   MergeParams mp{};

@@ -73,13 +73,18 @@ struct RoundState {
   // Groups of kSplitG+ members are split over workgroups (see kSplitGate): the group
   // kernel stages them (front), the pair kernel expands their bucket pairs
   // in chunks, the top-k kernel ranks their rows in chunks and the
-  // workgroup that finishes a group's last row chunk scores and selects it.
+  // workgroup that finishes a group's last row chunk lists its candidates;
+  // the scoring kernel scores them in chunks and the workgroup finishing the
+  // last chunk selects (with the learned ranker on, the lister scores and
+  // selects).
   int nsplit;        // split groups (records) of this round
   int npair_items;   // bucket-pair chunks queued
   int nrow_items;    // row chunks queued
   int steal_pairs;   // pair kernel work counter
   int steal_rows;    // top-k kernel work counter
-  int pad[3];
+  int nscore_items;  // exact-scoring chunks queued (plain method)
+  int steal_score;   // scoring kernel work counter
+  int pad[1];
 };
 
 // Groups of at least kSplitG members (the default threshold) are split, in
@@ -103,6 +108,9 @@ struct SplitRec {
   unsigned long long base;  // offset of the group's arrays in the scratch pool
   int rows_done;            // row chunks finished
   int ok;                   // 1 once the front has finished
+  uint32_t C;               // candidates listed
+  int score_chunks;         // scoring chunks queued
+  int score_done;           // scoring chunks finished
 };
 
 struct MergeParams {
@@ -141,6 +149,7 @@ struct MergeParams {
   SplitRec* split;
   int2* pair_items;         // (record, chunk)
   int2* row_items;          // (record, chunk); record < 0: unused
+  int2* score_items;        // (record, chunk) of candidates; nullptr: the lister scores
   int split_cap;            // records
   int split_min;            // groups of at least this many members are split
   int item_cap;             // entries of each item list
@@ -238,6 +247,7 @@ __global__ void MergeGroupsRankerKernel(MergeParams p);  // with the learned ran
 __global__ void SplitPairsKernel(MergeParams p);         // bucket-pair chunks of split groups
 __global__ void SplitRowsKernel(MergeParams p);          // top-k row chunks, then the group's scoring
 __global__ void SplitRowsRankerKernel(MergeParams p);
+__global__ void SplitScoreKernel(MergeParams p);         // exact-scoring chunks, then the group's selection
 __global__ void CommitKernel(CommitParams p);
 __global__ void TouchRound(RoundParams p);
 __global__ void FindCommitBlocks(RoundParams p);

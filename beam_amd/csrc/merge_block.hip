@@ -93,6 +93,14 @@ struct GroupArrays {
   long long* maoc;
   int32_t* mrep;
   int32_t* msize;
+  // Split groups scored in chunks: per member the row length, internal
+  // count, special-entry offset and count, and the sparse / dense sums.
+  int32_t* mlen;
+  int32_t* mint;
+  uint32_t* meoff;
+  uint32_t* mnspec;
+  long long* mssp;
+  long long* mbden;
 };
 
 
@@ -159,7 +167,7 @@ __device__ __forceinline__ unsigned long long LargeBytes(uint32_t E, uint32_t TC
   const unsigned long long e64 = E, tc64 = TC, g64 = static_cast<unsigned long long>(g), c64 = cmax;
   return e64 * 4 * 8 + tc64 * 4 + (tc64 + 1) * 4 + tc64 * 4 + 16 + tc64 * 16 + (tc64 + 1) * 4 + 16 +
          g64 * g64 * 8 + (g64 * g64 / 32 + 1) * 4 + 16 + c64 * (8 + 8 + 2 + 2) + (c64 + 1) * 4 + g64 * 64 + 16 +
-         e64 * 16 + 64 + (kBlkMaxBytes + 16) + g64 * 16 + 16;
+         e64 * 16 + 64 + (kBlkMaxBytes + 16) + g64 * 16 + 16 + g64 * 32 + 16;
 }
 
 __device__ __forceinline__ void LargeLayout(unsigned char* r, uint32_t E, uint32_t TC, int g, uint32_t cmax,
@@ -199,7 +207,14 @@ __device__ __forceinline__ void LargeLayout(unsigned char* r, uint32_t E, uint32
   r = align8(r);
   A.maoc = reinterpret_cast<long long*>(r); r += static_cast<unsigned long long>(g) * 8;
   A.mrep = reinterpret_cast<int32_t*>(r); r += static_cast<unsigned long long>(g) * 4;
-  A.msize = reinterpret_cast<int32_t*>(r);
+  A.msize = reinterpret_cast<int32_t*>(r); r += static_cast<unsigned long long>(g) * 4;
+  r = align8(r);
+  A.mssp = reinterpret_cast<long long*>(r); r += static_cast<unsigned long long>(g) * 8;
+  A.mbden = reinterpret_cast<long long*>(r); r += static_cast<unsigned long long>(g) * 8;
+  A.mlen = reinterpret_cast<int32_t*>(r); r += static_cast<unsigned long long>(g) * 4;
+  A.mint = reinterpret_cast<int32_t*>(r); r += static_cast<unsigned long long>(g) * 4;
+  A.meoff = reinterpret_cast<uint32_t*>(r); r += static_cast<unsigned long long>(g) * 4;
+  A.mnspec = reinterpret_cast<uint32_t*>(r);
 }
 
 // Bucket pairs [p0, p1) of a large group (sweg.cpp:1863-1903): for two
@@ -550,6 +565,192 @@ enum : int {
   kModeBack = 2,   // split group: candidate listing, exact scoring and selection
 };
 constexpr uint32_t kPairChunk = 4096;  // bucket pairs per pair-kernel item
+constexpr uint32_t kScoreChunk = 64;   // candidates per scoring-kernel item
+
+// LDS arrays the selection works in (the caller's).
+struct SelectLds {
+  const int32_t* rep;          // member reps
+  const int32_t* len;          // member row lengths
+  unsigned long long* best;    // kMaxGroup words
+  unsigned char* used;         // kMaxGroup flags, zero on entry
+  unsigned long long* chosen;  // kMaxSelect keys
+  uint32_t* wave_tot;
+  uint32_t* counts;            // counts[0]: zero on entry
+  int* nchosen;
+  int* live;
+  unsigned long long* gain;
+  unsigned long long* work;
+  unsigned long long* entries;
+};
+
+// Threshold filter and greedy selection of a scored group's C candidates
+// (gains and befores in A), writing its selected pairs to sel_out and its
+// statistics to slot; survivor keys go to skey (skey_cap entries).  Returns
+// false after an overflow error (the workgroup then stops).  All threads.
+__device__ __forceinline__ bool SelectGroup(const MergeParams& p, int g, int slot, int32_t* sel_out,
+                                            const GroupArrays& A, unsigned long long* skey, uint32_t skey_cap,
+                                            uint32_t C, uint32_t C0, uint32_t PT, const SelectLds& L,
+                                            uint32_t* P_out) {
+  const int tid = static_cast<int>(threadIdx.x);
+  const int32_t* m_rep = L.rep;
+  const int32_t* m_len = L.len;
+  unsigned char* s_used = L.used;
+  unsigned long long* s_chosen = L.chosen;
+  uint32_t* wave_tot = L.wave_tot;
+  uint32_t* s_counts = L.counts;
+  int& s_nchosen = *L.nchosen;
+  int& s_live = *L.live;
+  unsigned long long& s_gain = *L.gain;
+  unsigned long long& s_work = *L.work;
+  unsigned long long& s_entries = *L.entries;
+  // Phase 10: threshold filter (sweg.cpp:2146-2186) with the reference's FP64
+  // ratio.  Each survivor gets the selection key
+  //   (2^32 - 1 - gain) << 18 | a_idx << 9 | b_idx,
+  // ascending in (gain desc, a_idx asc, b_idx asc) (sweg.cpp:2217-2235); the
+  // gain is below 2^32 (it is at most `before`, at most 1.5 x arcs).
+  auto passes = [&](uint32_t pi) {
+    const long long gn = A.cand_gain[pi], bf = A.cand_before[pi];
+    return gn > 0 && bf > 0 && !(static_cast<double>(gn) / static_cast<double>(bf) < p.theta);
+  };
+  for (uint32_t pi = tid; pi < C; pi += kThreads) {
+    if (A.cand_gain[pi] > 0) atomicAdd(&s_counts[0], 1U);
+    A.pass[pi] = passes(pi) ? 1U : 0U;
+  }
+  if (tid == 0) {
+    s_nchosen = 0;
+    s_gain = 0;
+    s_work = 0;
+    s_entries = 0;
+  }
+  __syncthreads();
+  const uint32_t P = ScanInPlace(A.pass, static_cast<int>(C), wave_tot);
+  *P_out = P;
+  if (P > skey_cap) {
+    // Only reachable with --top-k above 16 on groups of more than 64 members.
+    if (tid == 0) atomicOr(p.err, kErrSelection);
+    return false;
+  }
+  for (uint32_t pi = tid; pi < C; pi += kThreads) {
+    if (A.pass[pi + 1] != A.pass[pi]) {
+      const unsigned long long gn = static_cast<unsigned long long>(A.cand_gain[pi]);
+      if (gn >> 32) atomicOr(p.err, kErrSelection);
+      skey[A.pass[pi]] = ((0xffffffffULL - gn) << 18) | (static_cast<unsigned long long>(A.cand_a[pi]) << 9) |
+                         static_cast<unsigned long long>(A.cand_b[pi]);
+    }
+  }
+  for (int i = tid; i < g; i += kThreads) atomicAdd(&s_entries, static_cast<unsigned long long>(m_len[i]));
+  __syncthreads();
+
+  // Phase 11: greedy endpoint-disjoint selection in key order
+  // (sweg.cpp:2237-2265), in rounds: a live pair whose key is the smallest
+  // among the live pairs at both of its endpoints is taken by the sequential
+  // greedy too (every better pair at its endpoints is already dead), and
+  // taking all such pairs at once leaves the greedy's choices unchanged.
+  // Pairs with a used endpoint die; survivors not taken are overlap rejects.
+  {
+    unsigned long long* best = L.best;
+    constexpr unsigned long long kDead = ~0ULL;
+    if (P > 0 && P <= static_cast<uint32_t>(kWave)) {
+      // One pair per lane of wave 0, wave-synchronous.
+      if (WaveId() == 0) {
+        const int lane = LaneId();
+        unsigned long long k = lane < static_cast<int>(P) ? skey[lane] : kDead;
+        const int a = static_cast<int>((k >> 9) & 511), b = static_cast<int>(k & 511);
+        while (__any(k != kDead)) {
+          if (k != kDead) {
+            best[a] = kDead;
+            best[b] = kDead;
+          }
+          WaveSync();
+          if (k != kDead) {
+            atomicMin(&best[a], k);
+            atomicMin(&best[b], k);
+          }
+          WaveSync();
+          if (k != kDead && best[a] == k && best[b] == k) {
+            s_used[a] = 1;
+            s_used[b] = 1;
+            s_chosen[atomicAdd(&s_nchosen, 1)] = k;
+          }
+          WaveSync();
+          if (k != kDead && (s_used[a] || s_used[b])) k = kDead;
+        }
+      }
+    } else if (P > 0) {
+      for (;;) {
+        for (uint32_t x = tid; x < P; x += kThreads) {
+          const unsigned long long k = skey[x];
+          if (k == kDead) continue;
+          best[(k >> 9) & 511] = kDead;
+          best[k & 511] = kDead;
+        }
+        if (tid == 0) s_live = 0;
+        __syncthreads();
+        for (uint32_t x = tid; x < P; x += kThreads) {
+          const unsigned long long k = skey[x];
+          if (k == kDead) continue;
+          atomicMin(&best[(k >> 9) & 511], k);
+          atomicMin(&best[k & 511], k);
+        }
+        __syncthreads();
+        for (uint32_t x = tid; x < P; x += kThreads) {
+          const unsigned long long k = skey[x];
+          if (k == kDead) continue;
+          const int a = static_cast<int>((k >> 9) & 511), b = static_cast<int>(k & 511);
+          if (best[a] == k && best[b] == k) {
+            s_used[a] = 1;
+            s_used[b] = 1;
+            s_chosen[atomicAdd(&s_nchosen, 1)] = k;
+          }
+        }
+        __syncthreads();
+        for (uint32_t x = tid; x < P; x += kThreads) {
+          const unsigned long long k = skey[x];
+          if (k == kDead) continue;
+          if (s_used[(k >> 9) & 511] || s_used[k & 511]) {
+            skey[x] = kDead;
+          } else {
+            s_live = 1;
+          }
+        }
+        __syncthreads();
+        if (!s_live) break;
+      }
+    }
+    __syncthreads();
+  }
+  // Selected pairs in key order; the keep is the member at the lower group
+  // position.
+  const int chosen = s_nchosen;
+  for (int c = tid; c < chosen; c += kThreads) {
+    const unsigned long long k = s_chosen[c];
+    int rank = 0;
+    for (int d = 0; d < chosen; ++d) rank += s_chosen[d] < k ? 1 : 0;
+    const int a = static_cast<int>((k >> 9) & 511), b = static_cast<int>(k & 511);
+    sel_out[2 * rank] = m_rep[a];
+    sel_out[2 * rank + 1] = m_rep[b];
+    atomicAdd(&s_gain, 0xffffffffULL - (k >> 18));
+    atomicAdd(&s_work, static_cast<unsigned long long>(m_len[a] + m_len[b]));
+  }
+  __syncthreads();
+  if (tid == 0) {
+    p.sel_count[slot] = chosen;
+    long long* st = p.slot_stats + static_cast<long long>(slot) * kSlotStats;
+    st[kStatGroup] = g;
+    st[kStatCand] = C0;
+    st[kStatScored] = C;
+    st[kStatProxy] = PT;
+    st[kStatPositive] = s_counts[0];
+    st[kStatPass] = P;
+    st[kStatOverlap] = static_cast<long long>(P) - chosen;
+    st[kStatChosen] = chosen;
+    st[kStatGain] = static_cast<long long>(s_gain);
+    st[kStatWork] = static_cast<long long>(s_work);
+    st[kStatEntries] = static_cast<long long>(s_entries);
+  }
+  __syncthreads();
+  return true;
+}
 
 // Candidate generation, exact scoring and selection for group gidx; results
 // go to sel_out (pairs), sel_count[slot], slot_stats[slot].  Every early exit
@@ -694,6 +895,9 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
     A.nz_pref = A.blk = nullptr;
     A.nz_info = nullptr;
     A.maoc = nullptr;
+    A.mlen = A.mint = nullptr;
+    A.meoff = A.mnspec = nullptr;
+    A.mssp = A.mbden = nullptr;
     A.sig = s_sig;
     A.spec = nullptr;
   } else {
@@ -1504,6 +1708,38 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
       }
     }
     __syncthreads();
+    if (mode == kModeBack && p.score_items != nullptr && C > 0) {
+      // Queue the candidates in chunks for the scoring kernel, with what
+      // scoring needs per member; without room they are scored here.
+      const int sc = static_cast<int>((C + kScoreChunk - 1) / kScoreChunk);
+      if (tid == 0) {
+        const int sb = atomicAdd(&p.round->nscore_items, sc);
+        s_flag = sb + sc <= p.item_cap ? 1 : 0;
+        s_pb = sb;
+      }
+      __syncthreads();
+      const int sb = s_pb;
+      const int ok = s_flag;
+      __syncthreads();
+      for (int c = tid; c < sc && sb + c < p.item_cap; c += kThreads)
+        p.score_items[sb + c] = make_int2(ok ? static_cast<int>(rec - p.split) : -1, c);
+      if (ok) {
+        for (int i = tid; i < g; i += kThreads) {
+          A.mssp[i] = m_ssp[i];
+          A.mbden[i] = m_bden[i];
+          A.mlen[i] = m_len[i];
+          A.mint[i] = m_int[i];
+          A.meoff[i] = m_eoff[i];
+          A.mnspec[i] = m_nspec[i];
+        }
+        if (tid == 0) {
+          rec->C = C;
+          rec->score_chunks = sc;
+          rec->score_done = 0;
+        }
+        return;
+      }
+    }
     for (uint32_t pi = tid; pi < C; pi += kThreads) {
       const int a = A.cand_a[pi], b = A.cand_b[pi];
       const int64_t sa = m_size[a], sb = m_size[b], sw = sa + sb;
@@ -1606,153 +1842,15 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
     if (tid == 0) p.dbg_count[slot] = static_cast<int>(C);
   }
 
-  // Phase 10: threshold filter (sweg.cpp:2146-2186) with the reference's FP64
-  // ratio.  Each survivor gets the selection key
-  //   (2^32 - 1 - gain) << 18 | a_idx << 9 | b_idx,
-  // ascending in (gain desc, a_idx asc, b_idx asc) (sweg.cpp:2217-2235); the
-  // gain is below 2^32 (it is at most `before`, at most 1.5 x arcs).
-  unsigned long long* skey = small ? reinterpret_cast<unsigned long long*>(A.tables)
-                                   : reinterpret_cast<unsigned long long*>(s_region);
-  auto passes = [&](uint32_t pi) {
-    const long long gn = A.cand_gain[pi], bf = A.cand_before[pi];
-    return gn > 0 && bf > 0 && !(static_cast<double>(gn) / static_cast<double>(bf) < p.theta);
-  };
-  for (uint32_t pi = tid; pi < C; pi += kThreads) {
-    if (A.cand_gain[pi] > 0) atomicAdd(&s_counts[0], 1U);
-    A.pass[pi] = passes(pi) ? 1U : 0U;
-  }
-  if (tid == 0) {
-    s_nchosen = 0;
-    s_gain = 0;
-    s_work = 0;
-    s_entries = 0;
-  }
-  __syncthreads();
-  const uint32_t P = ScanInPlace(A.pass, static_cast<int>(C), wave_tot);
-  if (P > static_cast<uint32_t>(small ? kTableBytes / 8 : kRegionBytes / 8)) {
-    // Only reachable with --top-k above 16 on groups of more than 64 members.
-    if (tid == 0) atomicOr(p.err, kErrSelection);
-    return;
-  }
-  for (uint32_t pi = tid; pi < C; pi += kThreads) {
-    if (A.pass[pi + 1] != A.pass[pi]) {
-      const unsigned long long gn = static_cast<unsigned long long>(A.cand_gain[pi]);
-      if (gn >> 32) atomicOr(p.err, kErrSelection);
-      skey[A.pass[pi]] = ((0xffffffffULL - gn) << 18) | (static_cast<unsigned long long>(A.cand_a[pi]) << 9) |
-                         static_cast<unsigned long long>(A.cand_b[pi]);
-    }
-  }
-  for (int i = tid; i < g; i += kThreads) atomicAdd(&s_entries, static_cast<unsigned long long>(m_len[i]));
-  __syncthreads();
-
-  // Phase 11: greedy endpoint-disjoint selection in key order
-  // (sweg.cpp:2237-2265), in rounds: a live pair whose key is the smallest
-  // among the live pairs at both of its endpoints is taken by the sequential
-  // greedy too (every better pair at its endpoints is already dead), and
-  // taking all such pairs at once leaves the greedy's choices unchanged.
-  // Pairs with a used endpoint die; survivors not taken are overlap rejects.
+  uint32_t P = 0;
   {
-    unsigned long long* best = reinterpret_cast<unsigned long long*>(m_aoc);  // free after top-k
-    constexpr unsigned long long kDead = ~0ULL;
-    if (P > 0 && P <= static_cast<uint32_t>(kWave)) {
-      // One pair per lane of wave 0, wave-synchronous.
-      if (WaveId() == 0) {
-        const int lane = LaneId();
-        unsigned long long k = lane < static_cast<int>(P) ? skey[lane] : kDead;
-        const int a = static_cast<int>((k >> 9) & 511), b = static_cast<int>(k & 511);
-        while (__any(k != kDead)) {
-          if (k != kDead) {
-            best[a] = kDead;
-            best[b] = kDead;
-          }
-          WaveSync();
-          if (k != kDead) {
-            atomicMin(&best[a], k);
-            atomicMin(&best[b], k);
-          }
-          WaveSync();
-          if (k != kDead && best[a] == k && best[b] == k) {
-            s_used[a] = 1;
-            s_used[b] = 1;
-            s_chosen[atomicAdd(&s_nchosen, 1)] = k;
-          }
-          WaveSync();
-          if (k != kDead && (s_used[a] || s_used[b])) k = kDead;
-        }
-      }
-    } else if (P > 0) {
-      for (;;) {
-        for (uint32_t x = tid; x < P; x += kThreads) {
-          const unsigned long long k = skey[x];
-          if (k == kDead) continue;
-          best[(k >> 9) & 511] = kDead;
-          best[k & 511] = kDead;
-        }
-        if (tid == 0) s_live = 0;
-        __syncthreads();
-        for (uint32_t x = tid; x < P; x += kThreads) {
-          const unsigned long long k = skey[x];
-          if (k == kDead) continue;
-          atomicMin(&best[(k >> 9) & 511], k);
-          atomicMin(&best[k & 511], k);
-        }
-        __syncthreads();
-        for (uint32_t x = tid; x < P; x += kThreads) {
-          const unsigned long long k = skey[x];
-          if (k == kDead) continue;
-          const int a = static_cast<int>((k >> 9) & 511), b = static_cast<int>(k & 511);
-          if (best[a] == k && best[b] == k) {
-            s_used[a] = 1;
-            s_used[b] = 1;
-            s_chosen[atomicAdd(&s_nchosen, 1)] = k;
-          }
-        }
-        __syncthreads();
-        for (uint32_t x = tid; x < P; x += kThreads) {
-          const unsigned long long k = skey[x];
-          if (k == kDead) continue;
-          if (s_used[(k >> 9) & 511] || s_used[k & 511]) {
-            skey[x] = kDead;
-          } else {
-            s_live = 1;
-          }
-        }
-        __syncthreads();
-        if (!s_live) break;
-      }
-    }
-    __syncthreads();
+    const SelectLds L{m_rep, m_len, reinterpret_cast<unsigned long long*>(m_aoc), s_used, s_chosen, wave_tot,
+                      s_counts, &s_nchosen, &s_live, &s_gain, &s_work, &s_entries};  // m_aoc is free after top-k
+    unsigned long long* skey = small ? reinterpret_cast<unsigned long long*>(A.tables)
+                                     : reinterpret_cast<unsigned long long*>(s_region);
+    const uint32_t cap = static_cast<uint32_t>(small ? kTableBytes / 8 : kRegionBytes / 8);
+    if (!SelectGroup(p, g, slot, sel_out, A, skey, cap, C, C0, PT, L, &P)) return;
   }
-  // Selected pairs in key order; the keep is the member at the lower group
-  // position.
-  const int chosen = s_nchosen;
-  for (int c = tid; c < chosen; c += kThreads) {
-    const unsigned long long k = s_chosen[c];
-    int rank = 0;
-    for (int d = 0; d < chosen; ++d) rank += s_chosen[d] < k ? 1 : 0;
-    const int a = static_cast<int>((k >> 9) & 511), b = static_cast<int>(k & 511);
-    sel_out[2 * rank] = m_rep[a];
-    sel_out[2 * rank + 1] = m_rep[b];
-    atomicAdd(&s_gain, 0xffffffffULL - (k >> 18));
-    atomicAdd(&s_work, static_cast<unsigned long long>(m_len[a] + m_len[b]));
-  }
-  __syncthreads();
-  if (tid == 0) {
-    p.sel_count[slot] = chosen;
-    long long* st = p.slot_stats + static_cast<long long>(slot) * kSlotStats;
-    st[kStatGroup] = g;
-    st[kStatCand] = C0;
-    st[kStatScored] = C;
-    st[kStatProxy] = PT;
-    st[kStatPositive] = s_counts[0];
-    st[kStatPass] = P;
-    st[kStatOverlap] = static_cast<long long>(P) - chosen;
-    st[kStatChosen] = chosen;
-    st[kStatGain] = static_cast<long long>(s_gain);
-    st[kStatWork] = static_cast<long long>(s_work);
-    st[kStatEntries] = static_cast<long long>(s_entries);
-  }
-  __syncthreads();
 
   stamp(12);
   if (trace != nullptr && tid == 0) {
@@ -1882,7 +1980,118 @@ __device__ __forceinline__ void SplitRowsBody(const MergeParams& p) {
   }
 }
 
+// Exact gains of candidates [pi0, pi0 + 8) of a split group on one wave:
+// eight lanes per candidate share the special entries of both members
+// (quotient_graph.cpp:340-462, as the group kernel's per-thread scoring: the
+// same integer terms, summed in another order).
+__device__ __forceinline__ void ScorePairsWave(const MergeParams& p, const GroupArrays& A, int g, uint32_t pi0,
+                                               uint32_t c1) {
+  const int lane = LaneId(), sub = lane & 7;
+  const uint32_t pi = pi0 + static_cast<uint32_t>(lane >> 3);
+  const bool have = pi < c1;
+  const uint32_t pc = have ? pi : pi0;
+  const int a = A.cand_a[pc], b = A.cand_b[pc];
+  const int64_t sa = A.msize[a], sb = A.msize[b], sw = sa + sb;
+  const int ra = A.mrep[a], rb = A.mrep[b];
+  int64_t before = 0, after = 0, eab[2] = {0, 0};
+#pragma unroll
+  for (int side = 0; side < 2; ++side) {
+    const int me = side == 0 ? a : b;
+    const int other_rep = side == 0 ? rb : ra;
+    const int64_t s_me = side == 0 ? sa : sb;
+    const uint4* spec4 = reinterpret_cast<const uint4*>(A.spec) + A.meoff[me];
+    const uint32_t cnt = have ? A.mnspec[me] : 0;
+#pragma unroll 2
+    for (uint32_t k = sub; k < cnt; k += 8) {
+      const uint4 w = spec4[k];
+      const int t = static_cast<int>(w.x);
+      const int64_t ee = static_cast<int64_t>(w.y), st = static_cast<int64_t>(w.z);
+      if (w.w != 0) {
+        if (t == other_rep) {
+          eab[side] = ee;
+          continue;
+        }
+        before += BlockCost(s_me, st, ee, false, p.err);
+      }
+      after += BlockCost(sw, st, ee, false, p.err);
+    }
+  }
+#pragma unroll
+  for (int off = 4; off > 0; off >>= 1) {  // sums over the candidate's eight lanes
+    before += __shfl_xor(before, off, kWave);
+    after += __shfl_xor(after, off, kWave);
+    eab[0] += __shfl_xor(eab[0], off, kWave);  // at most one lane holds each
+    eab[1] += __shfl_xor(eab[1], off, kWave);
+  }
+  if (have && sub == 0) {
+    if (eab[0] != eab[1]) atomicOr(p.err, kErrAsymmetric);
+    const int64_t ia = A.mint[a], ib = A.mint[b];
+    before += A.mssp[a] + A.mbden[a] + A.mssp[b] + A.mbden[b] + BlockCost(sa, sa, ia, true, p.err) +
+              BlockCost(sb, sb, ib, true, p.err) + BlockCost(sa, sb, eab[0], false, p.err);
+    after += A.mssp[a] + A.mssp[b] + CorrOf(A.proxy[a * g + b]) + BlockCost(sw, sw, ia + ib + eab[0], true, p.err);
+    // Agent-scope stores: coherent across the XCDs' L2s without a write-back.
+    __hip_atomic_store(&A.cand_gain[pi], before - after, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&A.cand_before[pi], before, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
 }  // namespace
+
+// Exact-scoring chunks of split groups (kScoreChunk candidates per item,
+// eight lanes per candidate); the workgroup that finishes a group's last
+// chunk (returning atomic count after its agent-scope gain stores are
+// performed, then an agent-scope acquire) runs the group's threshold filter
+// and selection.  Plain stores would not do: they can stay in the writing
+// XCD's L2, unseen by a finisher on another XCD.
+__global__ __launch_bounds__(kThreads) void SplitScoreKernel(MergeParams p) {
+  __shared__ __align__(16) unsigned char s_region[kRegionBytes];  // survivor keys
+  __shared__ int32_t t_rep[kMaxGroup], t_len[kMaxGroup];
+  __shared__ unsigned long long t_best[kMaxGroup];
+  __shared__ unsigned char t_used[kMaxGroup];
+  __shared__ unsigned long long t_chosen[kMaxSelect];
+  __shared__ uint32_t t_wave_tot[kWaves + 1], t_counts[2];
+  __shared__ int t_nchosen, t_live, s_t, s_last;
+  __shared__ unsigned long long t_gain, t_work, t_entries;
+  const int n = min(p.round->nscore_items, p.item_cap);
+  const int tid = static_cast<int>(threadIdx.x);
+  for (;;) {
+    if (tid == 0) s_t = atomicAdd(&p.round->steal_score, 1);
+    __syncthreads();
+    const int t = s_t;
+    __syncthreads();
+    if (t >= n) break;
+    const int2 it = p.score_items[t];
+    if (it.x < 0) continue;
+    SplitRec* rec = p.split + it.x;
+    GroupArrays A;
+    uint32_t TC;
+    const int g = Uniform(SplitArrays(p, *rec, &A, &TC));
+    const uint32_t C = Uniform(rec->C);
+    const uint32_t c0 = static_cast<uint32_t>(it.y) * kScoreChunk, c1 = min(C, c0 + kScoreChunk);
+    for (uint32_t pi = c0 + 8 * static_cast<uint32_t>(WaveId()); pi < c1; pi += 8 * kWaves) ScorePairsWave(p, A, g, pi, c1);
+    WaitWrites();
+    __syncthreads();
+    if (tid == 0) s_last = atomicAdd(&rec->score_done, 1) == rec->score_chunks - 1 ? 1 : 0;
+    __syncthreads();
+    if (s_last) {
+      FenceAcquireAgent();
+      __syncthreads();
+      for (int i = tid; i < g; i += kThreads) {
+        t_rep[i] = A.mrep[i];
+        t_len[i] = A.mlen[i];
+        t_used[i] = 0;
+      }
+      if (tid < 2) t_counts[tid] = 0;
+      __syncthreads();
+      const SelectLds L{t_rep, t_len, t_best, t_used, t_chosen, t_wave_tot,
+                        t_counts, &t_nchosen, &t_live, &t_gain, &t_work, &t_entries};
+      uint32_t P = 0;
+      SelectGroup(p, g, rec->gidx, p.sel + p.gstart[rec->gidx], A, reinterpret_cast<unsigned long long*>(s_region),
+                  static_cast<uint32_t>(kRegionBytes / 8), C, C, rec->PT, L, &P);
+      __syncthreads();
+    }
+  }
+}
 
 // Bucket-pair chunks of split groups (kPairChunk pairs per item).
 __global__ __launch_bounds__(kThreads) void SplitPairsKernel(MergeParams p) {
