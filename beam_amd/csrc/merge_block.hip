@@ -1109,12 +1109,12 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
     } else {
       // Phase 4, large groups: one pass over the slots numbers the buckets
       // with at least two entries (singleton buckets never pair).  Each wave
-      // takes 64 slots at a time, scans their pair and entry counts across
-      // its lanes and reserves bucket numbers, pairs and entry space with one
-      // LDS atomic; bucket number and first pair share one 64-bit counter
-      // (number << 34 | pairs), so the pair prefix ascends with the bucket
-      // number.  A bucket's slot count is replaced by its entry offset
-      // (kNoBucket for singletons).
+      // takes 256 slots at a time, four per lane (one 16-byte load), scans
+      // the lanes' pair and entry counts and reserves bucket numbers, pairs
+      // and entry space with one LDS atomic; bucket number and first pair
+      // share one 64-bit counter (number << 34 | pairs), so the pair prefix
+      // ascends with the bucket number.  A bucket's slot count is replaced
+      // by its entry offset (kNoBucket for singletons).
       constexpr uint32_t kNoBucket = 0xffffffffU;
       if (tid == 0) {
         s_pack = 0;
@@ -1124,41 +1124,74 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
       __syncthreads();
       {
         const int lane = LaneId();
-        const unsigned long long below = (1ULL << lane) - 1ULL;
-        for (uint32_t c0 = static_cast<uint32_t>(WaveId()) * kWave; c0 < TC; c0 += 4 * kThreads) {
-          uint32_t cv[4];
+        constexpr uint32_t kChunk = 4 * kWave;  // slots per wave step
+        constexpr uint32_t kStep = kWaves * kChunk;
+        // th_cnt is 16-byte aligned (32 E bytes and 4 TC bytes precede it)
+        // and TC is a power of two >= 64, so a lane's four slots are all in
+        // the table or all past it.
+        for (uint32_t c0 = static_cast<uint32_t>(WaveId()) * kChunk; c0 < TC; c0 += 2 * kStep) {
+          uint4 cv[2];
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            const uint32_t k = c0 + u * kThreads + lane;
-            cv[u] = k < TC ? A.th_cnt[k] : 0;
+          for (int u = 0; u < 2; ++u) {
+            const uint32_t k0 = c0 + u * kStep + 4 * lane;
+            cv[u] = k0 < TC ? *reinterpret_cast<const uint4*>(A.th_cnt + k0) : make_uint4(0, 0, 0, 0);
           }
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            const uint32_t k = c0 + u * kThreads + lane;
-            const uint32_t n = cv[u];
-            const bool nz = n >= 2;
-            const unsigned long long m = __ballot(nz);
-            if (n == 1) A.th_cnt[k] = kNoBucket;
-            if (m == 0) continue;
-            const uint32_t pr = nz ? n * (n - 1) / 2 : 0, en = nz ? n : 0;
-            const uint32_t pr_inc = WaveInclusiveScan(pr), en_inc = WaveInclusiveScan(en);
+          for (int u = 0; u < 2; ++u) {
+            const uint32_t k0 = c0 + u * kStep + 4 * lane;
+            uint32_t n4[4] = {cv[u].x, cv[u].y, cv[u].z, cv[u].w};
+            // Pairs, and entries | buckets << 20 (a bucket holds at most g
+            // <= 512 entries, so a wave's entries stay below 2^20).
+            uint32_t pr = 0, ennb = 0;
+            bool single = false;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+              const uint32_t n = n4[j];
+              if (n >= 2) {
+                pr += n * (n - 1) / 2;
+                ennb += n + (1U << 20);
+              }
+              single |= n == 1;
+            }
+            if (__ballot(ennb != 0) == 0) {
+              if (single) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) n4[j] = n4[j] == 1 ? kNoBucket : n4[j];
+                *reinterpret_cast<uint4*>(A.th_cnt + k0) = make_uint4(n4[0], n4[1], n4[2], n4[3]);
+              }
+              continue;
+            }
+            const uint32_t pr_inc = WaveInclusiveScan(pr), ennb_inc = WaveInclusiveScan(ennb);
             unsigned long long old = 0;
             uint32_t eold = 0;
             if (lane == kWave - 1) {
-              old = atomicAdd(&s_pack, (static_cast<unsigned long long>(__popcll(m)) << 34) + pr_inc);
-              eold = atomicAdd(&s_ents, en_inc);
+              old = atomicAdd(&s_pack, (static_cast<unsigned long long>(ennb_inc >> 20) << 34) + pr_inc);
+              eold = atomicAdd(&s_ents, ennb_inc & 0xfffffU);
               atomicAdd(&s_pt64, static_cast<unsigned long long>(pr_inc));
             }
             old = ReadLane64(old, kWave - 1);
             eold = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(eold), kWave - 1));
-            if (!nz) continue;
-            const uint32_t q = static_cast<uint32_t>(old >> 34) + static_cast<uint32_t>(__popcll(m & below));
-            const uint32_t eoff = eold + en_inc - en;
-            if (q < TC) {  // q wraps only when the pairs overflow, which the check below rejects
-              A.nz_info[q] = make_uint4(eoff, n, static_cast<uint32_t>(A.th_key[k]), static_cast<uint32_t>(A.th_st[k]));
-              A.nz_pref[q] = static_cast<uint32_t>(old) + pr_inc - pr;
+            uint32_t q = static_cast<uint32_t>(old >> 34) + ((ennb_inc - ennb) >> 20);
+            uint32_t pb = static_cast<uint32_t>(old) + pr_inc - pr;
+            uint32_t eoff = eold + ((ennb_inc - ennb) & 0xfffffU);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+              const uint32_t n = n4[j];
+              if (n >= 2) {
+                if (q < TC) {  // q wraps only when the pairs overflow, which the check below rejects
+                  A.nz_info[q] = make_uint4(eoff, n, static_cast<uint32_t>(A.th_key[k0 + j]),
+                                            static_cast<uint32_t>(A.th_st[k0 + j]));
+                  A.nz_pref[q] = pb;
+                }
+                n4[j] = eoff;
+                ++q;
+                pb += n * (n - 1) / 2;
+                eoff += n;
+              } else if (n == 1) {
+                n4[j] = kNoBucket;
+              }
             }
-            A.th_cnt[k] = eoff;
+            if (k0 < TC) *reinterpret_cast<uint4*>(A.th_cnt + k0) = make_uint4(n4[0], n4[1], n4[2], n4[3]);
           }
         }
       }
