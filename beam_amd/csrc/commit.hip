@@ -322,6 +322,49 @@ __global__ __launch_bounds__(kThreads) void CommitKernel(CommitParams p) {
 // so committing the leading such blocks together equals committing them one
 // after another (sweg.cpp:953-1145).
 
+// The selected pairs of one group on one wave, 64 at a time, lane k holding
+// pair k: its keep, remove, row offsets and lengths, and the inclusive
+// prefix of the pairs' entry counts (keep row, then remove row).
+struct WavePairs {
+  int keep, remove, lk;
+  uint32_t ok, orr, inc, len;
+};
+
+__device__ __forceinline__ WavePairs LoadWavePairs(const RoundParams& p, const int32_t* sp, int k, int cnt) {
+  WavePairs w{0, 0, 0, 0, 0, 0, 0};
+  if (k < cnt) {
+    w.keep = sp[2 * k];
+    w.remove = sp[2 * k + 1];
+    w.lk = p.q.row_len[w.keep];
+    w.len = static_cast<uint32_t>(w.lk + p.q.row_len[w.remove]);
+    w.ok = p.q.row_off[w.keep];
+    w.orr = p.q.row_off[w.remove];
+  }
+  w.inc = WaveInclusiveScan(w.len);
+  return w;
+}
+
+// Entry e (below the wave's total) of the pairs held by the wave: the pair's
+// lane, and whether the entry is in the remove row.  All lanes call it.
+__device__ __forceinline__ int WavePairEntry(const RoundParams& p, const WavePairs& w, uint32_t e, bool* in_remove,
+                                             int* owner) {
+  // The first lane whose inclusive prefix exceeds e.
+  int lo = 0;
+#pragma unroll
+  for (int step = kWave / 2; step > 0; step >>= 1) {
+    const uint32_t v = static_cast<uint32_t>(__shfl(static_cast<int>(w.inc), lo + step - 1, kWave));
+    if (v <= e) lo += step;
+  }
+  const uint32_t start = static_cast<uint32_t>(__shfl(static_cast<int>(w.inc - w.len), lo, kWave));
+  const int lk = __shfl(w.lk, lo, kWave);
+  const uint32_t ok = static_cast<uint32_t>(__shfl(static_cast<int>(w.ok), lo, kWave));
+  const uint32_t orr = static_cast<uint32_t>(__shfl(static_cast<int>(w.orr), lo, kWave));
+  const uint32_t idx = e - start;
+  *in_remove = idx >= static_cast<uint32_t>(lk);
+  *owner = lo;
+  return *in_remove ? p.q.pool[orr + idx - lk].nbr : p.q.pool[ok + idx].nbr;
+}
+
 // Records, per node, the first horizon block whose selected pairs touch it.
 __global__ __launch_bounds__(kThreads) void TouchRound(RoundParams p) {
   const int cursor = p.cur->cursor, end = p.cur->end;
@@ -344,14 +387,15 @@ __global__ __launch_bounds__(kThreads) void TouchRound(RoundParams p) {
     const unsigned long long val = (static_cast<unsigned long long>(p.epoch) << 16) |
                                    static_cast<unsigned long long>(0xffff - (gi / p.block_size - cursor));
     const int32_t* sp = p.sel + p.gstart[gi];
-    for (int k = 0; k < cnt; ++k) {
-      const int keep = sp[2 * k], remove = sp[2 * k + 1];
-      const int lk = p.q.row_len[keep], lr = p.q.row_len[remove];
-      const Entry* rk = p.q.pool + p.q.row_off[keep];
-      const Entry* rr = p.q.pool + p.q.row_off[remove];
-      for (int e = lane; e < lk + lr; e += kWave) {
-        const int x = e < lk ? rk[e].nbr : rr[e - lk].nbr;
-        atomicMax(&p.touch[x], val);
+    for (int k0 = 0; k0 < cnt; k0 += kWave) {
+      const WavePairs w = LoadWavePairs(p, sp, k0 + lane, cnt);
+      const uint32_t total = static_cast<uint32_t>(__shfl(static_cast<int>(w.inc), kWave - 1, kWave));
+      for (uint32_t e0 = 0; e0 < total; e0 += kWave) {
+        bool in_remove;
+        int owner;
+        const uint32_t e = min(e0 + static_cast<uint32_t>(lane), total - 1);
+        const int x = WavePairEntry(p, w, e, &in_remove, &owner);
+        if (e0 + static_cast<uint32_t>(lane) < total) atomicMax(&p.touch[x], val);
       }
     }
   }
@@ -410,24 +454,27 @@ __global__ __launch_bounds__(kThreads) void MarkRound(RoundParams p) {
     if (p.glen[gi] < 2) continue;
     const int cnt = p.sel_count[gi];
     const int32_t* sp = p.sel + p.gstart[gi];
-    for (int k = 0; k < cnt; ++k) {
-      const int keep = sp[2 * k], remove = sp[2 * k + 1];
-      if (lane == 0) {
-        p.mark_dest[remove] = p.epoch;
-        p.dest[remove] = keep;
-        p.mark_keep[keep] = p.epoch;
-        p.parent[remove] = keep;
-        p.merge_seq[remove] = p.epoch;
+    for (int k0 = 0; k0 < cnt; k0 += kWave) {
+      const WavePairs w = LoadWavePairs(p, sp, k0 + lane, cnt);
+      if (k0 + lane < cnt) {
+        p.mark_dest[w.remove] = p.epoch;
+        p.dest[w.remove] = w.keep;
+        p.mark_keep[w.keep] = p.epoch;
+        p.parent[w.remove] = w.keep;
+        p.merge_seq[w.remove] = p.epoch;
       }
-      const int lk = p.q.row_len[keep], lr = p.q.row_len[remove];
-      const Entry* rk = p.q.pool + p.q.row_off[keep];
-      const Entry* rr = p.q.pool + p.q.row_off[remove];
-      for (int e = lane; e < lk + lr; e += kWave) {
-        const int x = e < lk ? rk[e].nbr : rr[e - lk].nbr;
+      const uint32_t total = static_cast<uint32_t>(__shfl(static_cast<int>(w.inc), kWave - 1, kWave));
+      for (uint32_t e0 = 0; e0 < total; e0 += kWave) {
+        bool in_remove;
+        int owner;
+        const uint32_t e = min(e0 + static_cast<uint32_t>(lane), total - 1);
+        const int x = WavePairEntry(p, w, e, &in_remove, &owner);
+        const int keep = __shfl(w.keep, owner, kWave);
+        if (e0 + static_cast<uint32_t>(lane) >= total) continue;
         const int gx = p.node_group[x];
         const int bx = gx / bs;
         if (bx >= ncur && bx < nend && p.glen[gx] >= 2) queue(gx);
-        if (e >= lk && x != keep && atomicExch(&p.mark_row[x], p.epoch) != p.epoch) {
+        if (in_remove && x != keep && atomicExch(&p.mark_row[x], p.epoch) != p.epoch) {
           p.aff[atomicAdd(p.naff, 1)] = x;
         }
       }
