@@ -1336,12 +1336,17 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
     {
       const int wave = WaveId(), lane = LaneId();
       const int per_lane = (g + kWave - 1) / kWave;  // <= 8
+      // The kept-pair bits are in LDS here (named as such, so the accesses
+      // compile to LDS instructions rather than generic ones).
+      uint32_t* cbits = small ? s_candbits : reinterpret_cast<uint32_t*>(s_region);
       // Row i's proxies; the next row's are loaded while this one is ranked.
       auto load_row = [&](int ii, long long* dst) {
 #pragma unroll
         for (int r = 0; r < 8; ++r) {
           const int j = lane + r * kWave;
-          dst[r] = ii < g && r < per_lane && j < g && j != ii ? ProxyOf(A.proxy[min(ii, j) * g + max(ii, j)]) : 0;
+          const bool have = ii < g && r < per_lane && j < g && j != ii;
+          const int idx = min(ii, j) * g + max(ii, j);
+          dst[r] = !have ? 0 : ProxyOf(small ? s_proxy[idx] : A.proxy[idx]);
         }
       };
       long long pn[8];
@@ -1371,12 +1376,12 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
           if (h0 && b0 < topk) {
             const int j = wj[lane];
             const uint32_t bit = static_cast<uint32_t>(min(i, j)) * g + static_cast<uint32_t>(max(i, j));
-            atomicOr(&A.candbits[bit >> 5], 1U << (bit & 31));
+            atomicOr(&cbits[bit >> 5], 1U << (bit & 31));
           }
           if (h1 && b1 < topk) {
             const int j = wj[lane + kWave];
             const uint32_t bit = static_cast<uint32_t>(min(i, j)) * g + static_cast<uint32_t>(max(i, j));
-            atomicOr(&A.candbits[bit >> 5], 1U << (bit & 31));
+            atomicOr(&cbits[bit >> 5], 1U << (bit & 31));
           }
           WaveSync();
         };
@@ -1390,7 +1395,7 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
             const int j = lane + r * kWave;
             if (pw[r] > 0) {
               const uint32_t bit = static_cast<uint32_t>(min(i, j)) * g + static_cast<uint32_t>(max(i, j));
-              atomicOr(&A.candbits[bit >> 5], 1U << (bit & 31));
+              atomicOr(&cbits[bit >> 5], 1U << (bit & 31));
             }
           }
           continue;
@@ -1544,7 +1549,7 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
             live &= ~(1U << bslot);
             const int j = lane + bslot * kWave;
             const uint32_t bit = static_cast<uint32_t>(min(i, j)) * g + static_cast<uint32_t>(max(i, j));
-            atomicOr(&A.candbits[bit >> 5], 1U << (bit & 31));
+            atomicOr(&cbits[bit >> 5], 1U << (bit & 31));
           }
         }
       }
@@ -1560,15 +1565,18 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
   // its pairs in bit order.
   uint32_t C;
   {
+    // The bitmap is in LDS in every mode here (small: s_candbits; large
+    // and split groups: the region).
+    const uint32_t* cbits = small ? s_candbits : reinterpret_cast<const uint32_t*>(s_region);
     const uint32_t nw = (static_cast<uint32_t>(g) * static_cast<uint32_t>(g) + 31) / 32;
     const uint32_t per = (nw + kThreads - 1) / kThreads;
     const uint32_t w0 = min(nw, static_cast<uint32_t>(tid) * per), w1 = min(nw, w0 + per);
     uint32_t cnt = 0;
-    for (uint32_t k = w0; k < w1; ++k) cnt += static_cast<uint32_t>(__popc(A.candbits[k]));
+    for (uint32_t k = w0; k < w1; ++k) cnt += static_cast<uint32_t>(__popc(cbits[k]));
     uint32_t total = 0;
     uint32_t pos = BlockExclusiveScan(cnt, wave_tot, &total);
     for (uint32_t k = w0; k < w1; ++k) {
-      uint32_t w = A.candbits[k];
+      uint32_t w = cbits[k];
       while (w != 0) {
         const uint32_t bit = k * 32 + static_cast<uint32_t>(__ffs(w) - 1);
         w &= w - 1;
