@@ -233,7 +233,7 @@ class Engine:
 
     def set_split(self, min_members: int) -> None:
         """Groups of at least min_members members are split over workgroups (0: never;
-        default 65), in iterations whose largest group has max(min_members, 128)+ members."""
+        default 65), in iterations whose largest group has max(min_members, 200)+ members."""
         _check(self.lib.beam_hip_set_split(self.h, min_members))
 
     def kernel_profile(self) -> dict:

@@ -92,7 +92,7 @@ struct RoundState {
 // members: in iterations with only medium groups the two extra launches per
 // round cost more than they save.
 constexpr int kSplitG = 65;
-constexpr int kSplitGate = 128;
+constexpr int kSplitGate = 200;
 
 // One split group of a round (written by its front, read by the kernels
 // after it).

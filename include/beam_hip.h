@@ -143,7 +143,7 @@ int beam_hip_profile(beam_hip_engine* engine, int32_t enable);
 
 /* Groups of at least min_members members are split over the pair and row
  * kernels (default 65; 0 never splits), in iterations whose largest group
- * has at least max(min_members, 128) members.  No reference counterpart: a
+ * has at least max(min_members, 200) members.  No reference counterpart: a
  * tuning knob whose every value gives the same partition, counters and
  * output, which the tests check. */
 int beam_hip_set_split(beam_hip_engine* engine, int32_t min_members);
