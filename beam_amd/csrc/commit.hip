@@ -296,13 +296,24 @@ __global__ __launch_bounds__(kThreads) void CommitKernel(CommitParams p) {
   const int nslots = *p.nslots;
   const int first = p.first_group != nullptr ? *p.first_group : 0;
   const int32_t* gbase = p.first_group != nullptr ? p.gstart + first : nullptr;
-  SlotPrefix(p.sel_count + first, nslots, sh.slot_off, wave_tot);
-  const uint32_t npairs = sh.slot_off[nslots];
+  uint32_t npairs;
+  if (p.pairs != nullptr) {
+    npairs = static_cast<uint32_t>(*p.npairs);
+  } else {
+    SlotPrefix(p.sel_count + first, nslots, sh.slot_off, wave_tot);
+    npairs = sh.slot_off[nslots];
+  }
   const uint32_t naff = static_cast<uint32_t>(*p.naff);
   for (uint32_t item = blockIdx.x; item < npairs + naff; item += gridDim.x) {
     if (item < npairs) {
       int slot, keep, remove;
-      WindowPair(p.sel, sh.slot_off, nslots, item, gbase, &slot, &keep, &remove);
+      if (p.pairs != nullptr) {
+        const int2 pr = p.pairs[item];
+        keep = pr.x;
+        remove = pr.y;
+      } else {
+        WindowPair(p.sel, sh.slot_off, nslots, item, gbase, &slot, &keep, &remove);
+      }
       if (!UnionPair(p, sh, keep, remove)) return;
     } else {
       const int x = p.aff[item - npairs];
@@ -375,6 +386,7 @@ __global__ __launch_bounds__(kThreads) void TouchRound(RoundParams p) {
     p.next->nsplit = p.next->npair_items = p.next->nrow_items = p.next->steal_pairs = p.next->steal_rows = 0;
     p.next->nscore_items = p.next->steal_score = 0;
     *p.union_top = 0;
+    p.commit_out[3] = 0;
     p.round_ctr[1] += static_cast<unsigned long long>(p.cur->ntodo);
   }
   const int first = cursor * p.block_size;
@@ -403,23 +415,24 @@ __global__ __launch_bounds__(kThreads) void TouchRound(RoundParams p) {
 
 // j = the first horizon block (after the cursor's) with a member touched by
 // an earlier block of the horizon; blocks cursor .. cursor + j - 1 commit.
-__global__ __launch_bounds__(256) void FindCommitBlocks(RoundParams p) {
+// One wave per group, its lanes over the members.
+__global__ __launch_bounds__(kThreads) void FindCommitBlocks(RoundParams p) {
   const int cursor = p.cur->cursor, end = p.cur->end;
   const int first = (cursor + 1) * p.block_size;
   const int last = min(p.ngroups, end * p.block_size);
-  if (first >= last) return;
-  const int pos0 = p.gstart[first];
-  const int pos1 = p.gstart[last - 1] + p.glen[last - 1];
-  const int stride = static_cast<int>(gridDim.x * blockDim.x);
-  for (int pos = pos0 + static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x); pos < pos1; pos += stride) {
-    const int x = p.order[pos];
-    const int gi = p.node_group[x];
-    if (p.glen[gi] < 2) continue;
+  const int lane = LaneId();
+  const int waves = static_cast<int>(gridDim.x) * (kThreads / kWave);
+  for (int gi = first + static_cast<int>(blockIdx.x) * (kThreads / kWave) + WaveId(); gi < last; gi += waves) {
+    const int len = p.glen[gi];
+    if (len < 2) continue;
     const int rel = gi / p.block_size - cursor;
-    const unsigned long long t = p.touch[x];
-    if (static_cast<int>(t >> 16) == p.epoch && 0xffff - static_cast<int>(t & 0xffff) < rel) {
-      atomicMin(&p.cur->j, rel);
+    const int32_t* members = p.order + p.gstart[gi];
+    bool hit = false;
+    for (int k = lane; k < len; k += kWave) {
+      const unsigned long long t = p.touch[members[k]];
+      hit |= static_cast<int>(t >> 16) == p.epoch && 0xffff - static_cast<int>(t & 0xffff) < rel;
     }
+    if (__ballot(hit) != 0 && lane == 0) atomicMin(&p.cur->j, rel);
   }
 }
 
@@ -456,7 +469,11 @@ __global__ __launch_bounds__(kThreads) void MarkRound(RoundParams p) {
     const int32_t* sp = p.sel + p.gstart[gi];
     for (int k0 = 0; k0 < cnt; k0 += kWave) {
       const WavePairs w = LoadWavePairs(p, sp, k0 + lane, cnt);
+      int base = 0;
+      if (lane == 0) base = atomicAdd(&p.commit_out[3], min(kWave, cnt - k0));
+      base = __shfl(base, 0, kWave);
       if (k0 + lane < cnt) {
+        p.commit_pairs[base + lane] = make_int2(w.keep, w.remove);
         p.mark_dest[w.remove] = p.epoch;
         p.dest[w.remove] = w.keep;
         p.mark_keep[w.keep] = p.epoch;

@@ -459,7 +459,8 @@ struct Engine {
   int32_t* sel_count = nullptr;
   long long* slot_stats = nullptr;
   unsigned long long* touch = nullptr;
-  int32_t* commit_out = nullptr;  // [0] committed slots, [1] committed blocks, [2] first group
+  int32_t* commit_out = nullptr;  // [0] committed slots, [1] committed blocks, [2] first group, [3] pairs
+  int2* commit_pairs = nullptr;   // [n / 2 + 1] a round's committed pairs
   int32_t commit_host[2] = {0, 0};
 
   // Learned ranker (options.ranker).
@@ -579,7 +580,7 @@ Engine::~Engine() {
                   open_rep, mh,       recs[0],  recs[1],   flag,     excl,       rid,       run_start, run_open,
                   run_seg,  elem_open, new_idx, gflag,     gstart,   glen,       d_total,   d_maxlen, sel,
                   sel_count, mark_dest, dest,   mark_keep, mark_row, aff,        naff,      scratch_a, scratch_u,
-                  slot_stats, touch,  commit_out, node_group, gmark,     todo[0],   todo[1], rs, split, pair_items, row_items, score_items, round_ctr, samples, sample_target, sample_key,
+                  slot_stats, touch,  commit_out, commit_pairs, node_group, gmark,     todo[0],   todo[1], rs, split, pair_items, row_items, score_items, round_ctr, samples, sample_target, sample_key,
                   sample_count, audit_out, audit_list, audit_rs,
                   tops,     ctr,      err,      enc,        trace_buf};
   for (void* p : ptrs)
@@ -659,6 +660,7 @@ void Engine::Create(const int64_t* h_row_ptr, const int32_t* h_col, int n_, long
   slot_stats = DevAlloc<long long>(nslot * kSlotStats);
   touch = DevAlloc<unsigned long long>(n);
   commit_out = DevAlloc<int32_t>(4);
+  commit_pairs = DevAlloc<int2>(static_cast<size_t>(n) / 2 + 1);
   node_group = DevAlloc<int32_t>(n);
   gmark = DevAlloc<int32_t>(n);
   todo[0] = DevAlloc<int32_t>(n);
@@ -1025,6 +1027,7 @@ void Engine::LaunchRound(int G, int nblocks, int block_size, double theta, Count
   rp.aff = aff;
   rp.naff = &naff[epoch & 1];
   rp.commit_out = commit_out;
+  rp.commit_pairs = commit_pairs;
   rp.union_top = &tops[1];
   rp.round_ctr = round_ctr;
   rp.ctr = c;
@@ -1032,7 +1035,7 @@ void Engine::LaunchRound(int G, int nblocks, int block_size, double theta, Count
   const int slots = horizon_blocks * block_size;
   const int wave_grid = (slots + 7) / 8;  // one wave per group, 8 waves per workgroup
   TouchRound<<<wave_grid, 512, 0, stream>>>(rp);
-  FindCommitBlocks<<<256, 256, 0, stream>>>(rp);
+  FindCommitBlocks<<<wave_grid, 512, 0, stream>>>(rp);
   MarkRound<<<wave_grid, 512, 0, stream>>>(rp);
   Commit(c, true);
   ++windows_run;
@@ -1068,6 +1071,8 @@ void Engine::Commit(Counters* c, bool round) {
   cp.sel_count = sel_count;
   cp.nslots = &commit_out[0];
   cp.first_group = round ? &commit_out[2] : nullptr;
+  cp.pairs = round ? commit_pairs : nullptr;
+  cp.npairs = &commit_out[3];
   cp.gstart = gstart;
   cp.epoch = static_cast<int>(epoch);
   cp.mark_dest = mark_dest;

@@ -204,7 +204,8 @@ struct RoundParams {
   int32_t* mark_row;
   int32_t* aff;
   int32_t* naff;
-  int32_t* commit_out;        // [0] groups committed, [1] blocks, [2] first group
+  int32_t* commit_out;        // [0] groups committed, [1] blocks, [2] first group, [3] pairs
+  int2* commit_pairs;         // the committed pairs (keep, remove), listed by MarkRound
   unsigned long long* union_top;  // the commit's scratch top, zeroed by TouchRound
   unsigned long long* round_ctr;  // [0] rounds that committed blocks, [1] groups scored
   Counters* ctr;
@@ -222,6 +223,10 @@ struct CommitParams {
   const int32_t* sel;
   const int32_t* sel_count;
   const int32_t* nslots;       // slots to commit (device)
+  // Round mode: the committed pairs listed (pairs[0 .. *npairs)); window
+  // mode (pairs == nullptr): found through the slots' selection counts.
+  const int2* pairs;
+  const int32_t* npairs;
   // Round mode: groups first_group[0] + s; group s's pairs at sel[gstart[.]].
   const int32_t* first_group;
   const int32_t* gstart;
