@@ -1349,6 +1349,61 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
           dst[r] = !have ? 0 : ProxyOf(small ? s_proxy[idx] : A.proxy[idx]);
         }
       };
+      if (per_lane == 1) {
+        // At most 64 members: one partner per lane.  A row with more than k
+        // positive proxies sorts its keys (ratio bits, proxy << 32 | ~rep_j;
+        // zero for lanes without a positive proxy) descending across the
+        // wave with a bitonic network; the lanes holding the k best keys
+        // keep their partners -- the same k as the general ranking below
+        // (positive keys are distinct, so the order is total among them).
+        const int j = lane;
+        const int jj = min(j, g - 1);
+        const unsigned long long krep = 0xffffffffULL - static_cast<uint32_t>(m_rep[jj]);
+        const long long aoc_j = m_aoc[jj];
+        for (int i = wave; i < g; i += kWaves) {
+          const bool have = j < g && j != i;
+          const int idx = min(i, j) * g + max(i, j);
+          const long long v = have ? ProxyOf(small ? s_proxy[idx] : A.proxy[idx]) : 0;
+          const int cnt = __popcll(__ballot(v > 0));
+          int partner = j;
+          bool kept = v > 0;
+          if (cnt > topk) {
+            long long denom = m_aoc[i] + aoc_j;
+            if (denom < 1) denom = 1;
+            const double ratio = v > 0 ? static_cast<double>(v) / static_cast<double>(denom) : 0.0;
+            unsigned long long ka = v > 0 ? static_cast<unsigned long long>(__double_as_longlong(ratio)) : 0ULL;
+            unsigned long long kb = v > 0 ? (static_cast<unsigned long long>(v) << 32) | krep : 0ULL;
+#pragma unroll
+            for (int k = 2; k <= kWave; k <<= 1) {
+#pragma unroll
+              for (int st = k >> 1; st > 0; st >>= 1) {
+                const unsigned long long oa =
+                    (static_cast<unsigned long long>(static_cast<uint32_t>(__shfl_xor(static_cast<int>(ka >> 32), st, kWave))) << 32) |
+                    static_cast<uint32_t>(__shfl_xor(static_cast<int>(static_cast<uint32_t>(ka)), st, kWave));
+                const unsigned long long ob =
+                    (static_cast<unsigned long long>(static_cast<uint32_t>(__shfl_xor(static_cast<int>(kb >> 32), st, kWave))) << 32) |
+                    static_cast<uint32_t>(__shfl_xor(static_cast<int>(static_cast<uint32_t>(kb)), st, kWave));
+                const int op = __shfl_xor(partner, st, kWave);
+                // Descending segments where (lane & k) == 0 (all of the last
+                // stage); the lower lane of a pair keeps the larger key there.
+                const bool keep_max = ((lane & st) == 0) == ((lane & k) == 0);
+                const bool greater = oa > ka || (oa == ka && ob > kb);
+                const bool less = oa < ka || (oa == ka && ob < kb);
+                if (keep_max ? greater : less) {
+                  ka = oa;
+                  kb = ob;
+                  partner = op;
+                }
+              }
+            }
+            kept = lane < topk && ka != 0;
+          }
+          if (kept) {
+            const uint32_t bit = static_cast<uint32_t>(min(i, partner)) * g + static_cast<uint32_t>(max(i, partner));
+            atomicOr(&cbits[bit >> 5], 1U << (bit & 31));
+          }
+        }
+      } else {
       long long pn[8];
       load_row(wave, pn);
       for (int i = wave; i < g; i += kWaves) {
@@ -1552,6 +1607,7 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
             atomicOr(&cbits[bit >> 5], 1U << (bit & 31));
           }
         }
+      }
       }
     }
     __syncthreads();
