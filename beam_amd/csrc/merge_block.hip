@@ -14,6 +14,8 @@
 //                      coalescing with an existing keep entry.
 // Every group of a block reads the block-start state and commits happen after
 // all groups are scored, as in the reference (sweg.cpp:1097-1110, 2683-2747).
+#include <type_traits>
+
 #include "engine.hpp"
 #include "score_pair.hpp"
 
@@ -1263,38 +1265,79 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
       }
     }
     stamp(15);
+    // A front also builds its members' special-entry lists and sparse /
+    // dense sums for the exact scoring here (the member lookups are the
+    // direct-edge pass's), in LDS words of the free join tables.
+    long long* f_ssp = reinterpret_cast<long long*>(A.tables);
+    long long* f_bden = f_ssp + kMaxGroup;
+    uint32_t* f_nspec = reinterpret_cast<uint32_t*>(f_bden + kMaxGroup);
+    if (mode == kModeFront) {
+      for (int i = tid; i < g; i += kThreads) {
+        f_ssp[i] = 0;
+        f_bden[i] = 0;
+        f_nspec[i] = 0;
+      }
+      __syncthreads();
+    }
     // Phase 6: direct edges between members (sweg.cpp:1905-1924); four
     // entries per thread at a time, their loads in flight together.
-    for (uint32_t e0 = tid; e0 < E; e0 += 4 * kThreads) {
-      int t4[4], i4[4], sl4[4], c4[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const uint32_t e = min(e0 + u * kThreads, E - 1);
-        sl4[u] = e0 + u * kThreads < E ? A.ent_slot[e] : -1;
-        t4[u] = A.ent_t[e];
-        i4[u] = small ? s_ent_i[e] : A.g_ent_i[e];
-        c4[u] = A.ent_e[e];
-      }
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        if (sl4[u] < 0) continue;
-        const int t = t4[u];
-        uint32_t h = HashInt(static_cast<uint32_t>(t)) & (kMemberSlots - 1);
-        int j = -1;
-        while (true) {
-          const int key = mh_key[h];
-          if (key == t) {
-            j = mh_val[h];
-            break;
+    // Instantiated apart for fronts, so the other groups' loop is unchanged.
+    auto direct_edges = [&](auto front_tag) {
+      constexpr bool kFront = decltype(front_tag)::value;
+      for (uint32_t e0 = tid; e0 < E; e0 += 4 * kThreads) {
+        int t4[4], i4[4], sl4[4], c4[4], st4[4];
+  #pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const uint32_t e = min(e0 + u * kThreads, E - 1);
+          sl4[u] = e0 + u * kThreads < E ? A.ent_slot[e] : -1;
+          t4[u] = A.ent_t[e];
+          i4[u] = small ? s_ent_i[e] : A.g_ent_i[e];
+          c4[u] = A.ent_e[e];
+          if constexpr (kFront) st4[u] = A.ent_st[e];
+        }
+  #pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          if (sl4[u] < 0) continue;
+          const int t = t4[u];
+          uint32_t h = HashInt(static_cast<uint32_t>(t)) & (kMemberSlots - 1);
+          int j = -1;
+          while (true) {
+            const int key = mh_key[h];
+            if (key == t) {
+              j = mh_val[h];
+              break;
+            }
+            if (key == -1) break;
+            h = (h + 1) & (kMemberSlots - 1);
           }
-          if (key == -1) break;
-          h = (h + 1) & (kMemberSlots - 1);
-        }
-        if (j > i4[u]) {
-          atomicAdd(reinterpret_cast<unsigned long long*>(&A.proxy[i4[u] * g + j]),
-                    static_cast<unsigned long long>(c4[u]));
+          if (j > i4[u]) {
+            atomicAdd(reinterpret_cast<unsigned long long*>(&A.proxy[i4[u] * g + j]),
+                      static_cast<unsigned long long>(c4[u]));
+          }
+          if constexpr (kFront) {
+            // As the exact scoring's special-entry pass (phase 9).
+            const int i = i4[u];
+            const int64_t ee = c4[u], st = st4[u], sa = m_size[i];
+            const bool member = j >= 0;
+            if (member || 2 * ee > 1 + sa * st) {
+              const uint32_t pos = m_eoff[i] + atomicAdd(&f_nspec[i], 1U);
+              reinterpret_cast<uint4*>(A.spec)[pos] = make_uint4(static_cast<uint32_t>(t), static_cast<uint32_t>(ee),
+                                                                 static_cast<uint32_t>(st), member ? 1U : 0U);
+              if (!member) {
+                atomicAdd(reinterpret_cast<unsigned long long*>(&f_bden[i]),
+                          static_cast<unsigned long long>(BlockCost(sa, st, ee, false, p.err)));
+              }
+            } else {
+              atomicAdd(reinterpret_cast<unsigned long long*>(&f_ssp[i]), static_cast<unsigned long long>(ee));
+            }
+          }
         }
       }
+    };
+    if (mode == kModeFront) {
+      direct_edges(std::integral_constant<bool, true>{});
+    } else {
+      direct_edges(std::integral_constant<bool, false>{});
     }
     __syncthreads();
     if (mode == kModeFront) {
@@ -1304,6 +1347,9 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
         A.maoc[i] = m_aoc[i];
         A.mrep[i] = m_rep[i];
         A.msize[i] = m_size[i];
+        A.mssp[i] = f_ssp[i];
+        A.mbden[i] = f_bden[i];
+        A.mnspec[i] = f_nspec[i];
       }
       if (tid == 0) {
         rec->E = static_cast<int>(E);
@@ -1765,46 +1811,56 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
     uint32_t* m_nspec = reinterpret_cast<uint32_t*>(m_bden + kMaxGroup);
     uint16_t* spec16 = reinterpret_cast<uint16_t*>(m_nspec + kMaxGroup);
     constexpr uint32_t kMemberFlag16 = 0x8000U;
-    for (int i = tid; i < g; i += kThreads) {
-      m_ssp[i] = 0;
-      m_bden[i] = 0;
-      m_nspec[i] = 0;
-    }
-    __syncthreads();
-#pragma unroll 4
-    for (uint32_t e = tid; e < E; e += kThreads) {
-      if (A.ent_slot[e] < 0) continue;  // the doubled self entry is not a cross entry
-      const int i = small ? s_ent_i[e] : A.g_ent_i[e];
-      const int t = A.ent_t[e];
-      const int64_t ee = A.ent_e[e], st = A.ent_st[e], sa = m_size[i];
-      uint32_t h = HashInt(static_cast<uint32_t>(t)) & (kMemberSlots - 1);
-      bool member = false;
-      while (true) {
-        const int key = mh_key[h];
-        if (key == t) {
-          member = true;
-          break;
-        }
-        if (key == -1) break;
-        h = (h + 1) & (kMemberSlots - 1);
+    if (mode == kModeBack) {
+      // The front built them.
+      for (int i = tid; i < g; i += kThreads) {
+        m_ssp[i] = A.mssp[i];
+        m_bden[i] = A.mbden[i];
+        m_nspec[i] = A.mnspec[i];
       }
-      if (member || 2 * ee > 1 + sa * st) {
-        const uint32_t pos = m_eoff[i] + atomicAdd(&m_nspec[i], 1U);
-        if (small) {
-          spec16[pos] = static_cast<uint16_t>(e | (member ? kMemberFlag16 : 0U));
+      __syncthreads();
+    } else {
+      for (int i = tid; i < g; i += kThreads) {
+        m_ssp[i] = 0;
+        m_bden[i] = 0;
+        m_nspec[i] = 0;
+      }
+      __syncthreads();
+  #pragma unroll 4
+      for (uint32_t e = tid; e < E; e += kThreads) {
+        if (A.ent_slot[e] < 0) continue;  // the doubled self entry is not a cross entry
+        const int i = small ? s_ent_i[e] : A.g_ent_i[e];
+        const int t = A.ent_t[e];
+        const int64_t ee = A.ent_e[e], st = A.ent_st[e], sa = m_size[i];
+        uint32_t h = HashInt(static_cast<uint32_t>(t)) & (kMemberSlots - 1);
+        bool member = false;
+        while (true) {
+          const int key = mh_key[h];
+          if (key == t) {
+            member = true;
+            break;
+          }
+          if (key == -1) break;
+          h = (h + 1) & (kMemberSlots - 1);
+        }
+        if (member || 2 * ee > 1 + sa * st) {
+          const uint32_t pos = m_eoff[i] + atomicAdd(&m_nspec[i], 1U);
+          if (small) {
+            spec16[pos] = static_cast<uint16_t>(e | (member ? kMemberFlag16 : 0U));
+          } else {
+            reinterpret_cast<uint4*>(A.spec)[pos] = make_uint4(static_cast<uint32_t>(t), static_cast<uint32_t>(ee),
+                                                               static_cast<uint32_t>(st), member ? 1U : 0U);
+          }
+          if (!member) {
+            atomicAdd(reinterpret_cast<unsigned long long*>(&m_bden[i]),
+                      static_cast<unsigned long long>(BlockCost(sa, st, ee, false, p.err)));
+          }
         } else {
-          reinterpret_cast<uint4*>(A.spec)[pos] = make_uint4(static_cast<uint32_t>(t), static_cast<uint32_t>(ee),
-                                                             static_cast<uint32_t>(st), member ? 1U : 0U);
+          atomicAdd(reinterpret_cast<unsigned long long*>(&m_ssp[i]), static_cast<unsigned long long>(ee));
         }
-        if (!member) {
-          atomicAdd(reinterpret_cast<unsigned long long*>(&m_bden[i]),
-                    static_cast<unsigned long long>(BlockCost(sa, st, ee, false, p.err)));
-        }
-      } else {
-        atomicAdd(reinterpret_cast<unsigned long long*>(&m_ssp[i]), static_cast<unsigned long long>(ee));
       }
+      __syncthreads();
     }
-    __syncthreads();
     if (mode == kModeBack && p.score_items != nullptr && C > 0) {
       // Queue the candidates in chunks for the scoring kernel, with what
       // scoring needs per member; without room they are scored here.
@@ -1822,12 +1878,9 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
         p.score_items[sb + c] = make_int2(ok ? static_cast<int>(rec - p.split) : -1, c);
       if (ok) {
         for (int i = tid; i < g; i += kThreads) {
-          A.mssp[i] = m_ssp[i];
-          A.mbden[i] = m_bden[i];
           A.mlen[i] = m_len[i];
           A.mint[i] = m_int[i];
           A.meoff[i] = m_eoff[i];
-          A.mnspec[i] = m_nspec[i];
         }
         if (tid == 0) {
           rec->C = C;
