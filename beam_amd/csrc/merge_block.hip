@@ -754,6 +754,51 @@ __device__ __forceinline__ bool SelectGroup(const MergeParams& p, int g, int slo
   return true;
 }
 
+// Arrays of split record r's group (the scratch layout its front allocated).
+__device__ __forceinline__ int SplitArrays(const MergeParams& p, const SplitRec& r, GroupArrays* A, uint32_t* TC) {
+  const int g = p.glen[r.gidx];
+  const uint32_t E = static_cast<uint32_t>(r.E);
+  uint32_t tc = 64;
+  while (tc < 2 * E) tc <<= 1;
+  const int topk = max(1, min(p.top_k, g - 1));
+  const uint32_t cmax = min(static_cast<uint32_t>(g) * static_cast<uint32_t>(topk),
+                            static_cast<uint32_t>(g) * static_cast<uint32_t>(g - 1) / 2);
+  LargeLayout(p.scratch + r.base, E, tc, g, cmax, *A);
+  *TC = tc;
+  return g;
+}
+
+// Phase 8: candidate pairs in ascending (a_idx, b_idx) order
+// (sweg.cpp:1979-1989).  A kept pair a < b is bit a * g + b of the bitmap
+// `bits` (LDS), so that order is the order of the set bits: each thread
+// counts the bits of a run of words, a block scan places the runs, and each
+// thread writes its pairs in bit order.  Returns the candidate count.  All
+// threads of the workgroup.
+__device__ __forceinline__ uint32_t ListCandidates(const uint32_t* bits, int g, uint16_t* cand_a, uint16_t* cand_b,
+                                                   uint32_t* wave_tot) {
+  const uint32_t nw = (static_cast<uint32_t>(g) * static_cast<uint32_t>(g) + 31) / 32;
+  const uint32_t per = (nw + kThreads - 1) / kThreads;
+  const uint32_t w0 = min(nw, static_cast<uint32_t>(threadIdx.x) * per), w1 = min(nw, w0 + per);
+  uint32_t cnt = 0;
+  for (uint32_t k = w0; k < w1; ++k) cnt += static_cast<uint32_t>(__popc(bits[k]));
+  uint32_t total = 0;
+  uint32_t pos = BlockExclusiveScan(cnt, wave_tot, &total);
+  for (uint32_t k = w0; k < w1; ++k) {
+    uint32_t w = bits[k];
+    while (w != 0) {
+      const uint32_t bit = k * 32 + static_cast<uint32_t>(__ffs(w) - 1);
+      w &= w - 1;
+      const uint32_t lo = bit / static_cast<uint32_t>(g);
+      cand_a[pos] = static_cast<uint16_t>(lo);
+      cand_b[pos] = static_cast<uint16_t>(bit - lo * static_cast<uint32_t>(g));
+      ++pos;
+    }
+  }
+  const uint32_t C = Uniform(total);
+  __syncthreads();
+  return C;
+}
+
 // Candidate generation, exact scoring and selection for group gidx; results
 // go to sel_out (pairs), sel_count[slot], slot_stats[slot].  Every early exit
 // is taken by the whole workgroup.  In round mode with split records, a group
@@ -814,13 +859,51 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
   }
 
   __shared__ int32_t m_rep[kMaxGroup], m_size[kMaxGroup], m_len[kMaxGroup], m_int[kMaxGroup];
+  __shared__ uint32_t wave_tot[kWaves + 1];
+  __shared__ int s_flag;
+  if (mode == kModeBack && p.score_items != nullptr) {
+    // Plain method: the lister lists the candidates (from an LDS copy of the
+    // kept-pair bitmap) and queues their scoring; the front left everything
+    // the scoring and selection need in the scratch.  Without candidates or
+    // without room the full back below runs instead.
+    GroupArrays A;
+    uint32_t TC;
+    SplitArrays(p, *rec, &A, &TC);
+    uint32_t* bits = reinterpret_cast<uint32_t*>(s_region);
+    const uint32_t nw = (static_cast<uint32_t>(g) * static_cast<uint32_t>(g) + 31) / 32;
+#pragma unroll 4
+    for (uint32_t k = tid; k < nw; k += kThreads) bits[k] = A.candbits[k];
+    __syncthreads();
+    const uint32_t C = ListCandidates(bits, g, A.cand_a, A.cand_b, wave_tot);
+    if (C > 0) {
+      const int sc = static_cast<int>((C + kScoreChunk - 1) / kScoreChunk);
+      if (tid == 0) {
+        const int sb = atomicAdd(&p.round->nscore_items, sc);
+        s_flag = sb + sc <= p.item_cap ? 1 : 0;
+        s_pb = sb;
+      }
+      __syncthreads();
+      const int sb = s_pb;
+      const int ok = s_flag;
+      __syncthreads();
+      for (int c = tid; c < sc && sb + c < p.item_cap; c += kThreads)
+        p.score_items[sb + c] = make_int2(ok ? static_cast<int>(rec - p.split) : -1, c);
+      if (ok) {
+        if (tid == 0) {
+          rec->C = C;
+          rec->score_chunks = sc;
+          rec->score_done = 0;
+        }
+        return;
+      }
+    }
+  }
   __shared__ uint32_t m_off[kMaxGroup];
   __shared__ uint32_t m_eoff[kMaxGroup + 1];
   __shared__ uint32_t m_rowcnt[kMaxGroup + 1];
   __shared__ long long m_aoc[kMaxGroup];
   __shared__ int32_t mh_key[kMemberSlots];
   __shared__ int16_t mh_val[kMemberSlots];
-  __shared__ uint32_t wave_tot[kWaves + 1];
   __shared__ long long s_proxy[kSmallG * kSmallG];
   __shared__ uint32_t s_candbits[kSmallG * kSmallG / 32];
   __shared__ int32_t s_ent_t[kSmallE], s_ent_e[kSmallE], s_ent_st[kSmallE];
@@ -830,7 +913,6 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
   __shared__ unsigned long long s_gain, s_work, s_entries;
   __shared__ int s_nchosen, s_live;
   __shared__ unsigned long long s_base;
-  __shared__ int s_flag;
   __shared__ uint32_t s_counts[2];
   __shared__ uint32_t s_sig[kSmallG * 16];
   __shared__ uint32_t s_prefix, s_need;
@@ -1350,6 +1432,9 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
         A.mssp[i] = f_ssp[i];
         A.mbden[i] = f_bden[i];
         A.mnspec[i] = f_nspec[i];
+        A.mlen[i] = m_len[i];
+        A.mint[i] = m_int[i];
+        A.meoff[i] = m_eoff[i];
       }
       if (tid == 0) {
         rec->E = static_cast<int>(E);
@@ -1660,37 +1745,10 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
   }
 
   stamp(9);
-  // Phase 8: candidate pairs in ascending (a_idx, b_idx) order
-  // (sweg.cpp:1979-1989).  A kept pair a < b is bit a * g + b of the bitmap,
-  // so that order is the order of the set bits: each thread counts the bits
-  // of a run of words, a block scan places the runs, and each thread writes
-  // its pairs in bit order.
-  uint32_t C;
-  {
-    // The bitmap is in LDS in every mode here (small: s_candbits; large
-    // and split groups: the region).
-    const uint32_t* cbits = small ? s_candbits : reinterpret_cast<const uint32_t*>(s_region);
-    const uint32_t nw = (static_cast<uint32_t>(g) * static_cast<uint32_t>(g) + 31) / 32;
-    const uint32_t per = (nw + kThreads - 1) / kThreads;
-    const uint32_t w0 = min(nw, static_cast<uint32_t>(tid) * per), w1 = min(nw, w0 + per);
-    uint32_t cnt = 0;
-    for (uint32_t k = w0; k < w1; ++k) cnt += static_cast<uint32_t>(__popc(cbits[k]));
-    uint32_t total = 0;
-    uint32_t pos = BlockExclusiveScan(cnt, wave_tot, &total);
-    for (uint32_t k = w0; k < w1; ++k) {
-      uint32_t w = cbits[k];
-      while (w != 0) {
-        const uint32_t bit = k * 32 + static_cast<uint32_t>(__ffs(w) - 1);
-        w &= w - 1;
-        const uint32_t lo = bit / static_cast<uint32_t>(g);
-        A.cand_a[pos] = static_cast<uint16_t>(lo);
-        A.cand_b[pos] = static_cast<uint16_t>(bit - lo * static_cast<uint32_t>(g));
-        ++pos;
-      }
-    }
-    C = Uniform(total);
-    __syncthreads();
-  }
+  // Phase 8: the candidate list.  The bitmap is in LDS in every mode here
+  // (small: s_candbits; large and split groups: the region).
+  uint32_t C = ListCandidates(small ? s_candbits : reinterpret_cast<const uint32_t*>(s_region), g, A.cand_a,
+                              A.cand_b, wave_tot);
   const uint32_t C0 = C;
 
   // Learned ranker: features of a candidate pair from the staged group.
@@ -2060,20 +2118,6 @@ __global__ __launch_bounds__(kThreads) void MergeGroupsKernel(MergeParams p) { M
 __global__ __launch_bounds__(kThreads) void MergeGroupsRankerKernel(MergeParams p) { MergeGroupsBody<true>(p); }
 
 namespace {
-
-// Arrays of split record r's group (the scratch layout its front allocated).
-__device__ __forceinline__ int SplitArrays(const MergeParams& p, const SplitRec& r, GroupArrays* A, uint32_t* TC) {
-  const int g = p.glen[r.gidx];
-  const uint32_t E = static_cast<uint32_t>(r.E);
-  uint32_t tc = 64;
-  while (tc < 2 * E) tc <<= 1;
-  const int topk = max(1, min(p.top_k, g - 1));
-  const uint32_t cmax = min(static_cast<uint32_t>(g) * static_cast<uint32_t>(topk),
-                            static_cast<uint32_t>(g) * static_cast<uint32_t>(g - 1) / 2);
-  LargeLayout(p.scratch + r.base, E, tc, g, cmax, *A);
-  *TC = tc;
-  return g;
-}
 
 __device__ __forceinline__ void FenceAcquireAgent() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
 // This wave's outstanding stores and atomics performed (s_waitcnt vmcnt(0)).
