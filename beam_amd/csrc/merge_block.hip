@@ -110,6 +110,18 @@ struct GroupArrays {
 // Exclusive scan in place over n entries of `data` (generic pointer), with
 // data[n] receiving the total.  All threads of the workgroup participate.
 __device__ uint32_t ScanInPlace(uint32_t* data, int n, uint32_t* wave_tot) {
+  if (n <= kWave) {
+    // One wave: a shuffle scan and a single barrier.
+    if (threadIdx.x < kWave) {
+      const int lane = static_cast<int>(threadIdx.x);
+      const uint32_t v = lane < n ? data[lane] : 0U;
+      const uint32_t inc = WaveInclusiveScan(v);
+      if (lane < n) data[lane] = inc - v;
+      if (lane == kWave - 1) data[n] = inc;
+    }
+    __syncthreads();
+    return data[n];
+  }
   const int per = (n + kThreads - 1) / kThreads;
   const int begin = static_cast<int>(threadIdx.x) * per;
   const int end = min(n, begin + per);
