@@ -240,9 +240,8 @@ __device__ __forceinline__ void LargeLayout(unsigned char* r, uint32_t E, uint32
 // of the compacted buckets' pair prefix (nzp); four pairs per thread are in
 // flight together.  All threads of the workgroup.
 __device__ __forceinline__ void BucketPairs(const GroupArrays& A, const uint32_t* blk, const uint32_t* nzp,
-                                            const int32_t* msize, const int32_t* mrep, int g, uint32_t PT,
-                                            uint32_t NB, int shift, uint32_t nblk, uint32_t p0, uint32_t p1,
-                                            int* err) {
+                                            const int32_t* msize, const int32_t* mrep, int g, uint32_t NB,
+                                            int shift, uint32_t nblk, uint32_t p0, uint32_t p1, int* err) {
   constexpr int kU = 4;
   for (uint32_t base = p0 + threadIdx.x; base < p1; base += kU * kThreads) {
     uint32_t lo[kU], l[kU];
@@ -1355,7 +1354,7 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
             p.pair_items[pb + c] = make_int2(ok ? static_cast<int>(rec - p.split) : -1, c);
           inline_pairs = ok == 0;
         }
-        if (inline_pairs) BucketPairs(A, blk, nzp, m_size, m_rep, g, PT, NB, shift, nblk, 0, PT, p.err);
+        if (inline_pairs) BucketPairs(A, blk, nzp, m_size, m_rep, g, NB, shift, nblk, 0, PT, p.err);
       }
     }
     stamp(15);
@@ -2317,7 +2316,7 @@ __global__ __launch_bounds__(kThreads) void SplitPairsKernel(MergeParams p) {
     uint32_t TC;
     const int g = Uniform(SplitArrays(p, rec, &A, &TC));
     const uint32_t p0 = static_cast<uint32_t>(it.y) * kPairChunk;
-    BucketPairs(A, A.blk, A.nz_pref, A.msize, A.mrep, g, rec.PT, rec.NB, rec.shift, rec.nblk, p0,
+    BucketPairs(A, A.blk, A.nz_pref, A.msize, A.mrep, g, rec.NB, rec.shift, rec.nblk, p0,
                 min(rec.PT, p0 + kPairChunk), p.err);
   }
 }
