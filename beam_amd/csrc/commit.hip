@@ -15,6 +15,7 @@
 //             compacted).  Queued rows that are themselves endpoints of the
 //             block are skipped: their unions already map every entry.
 #include "engine.hpp"
+#include "score_pair.hpp"  // WaveSync
 
 namespace beam {
 
@@ -247,6 +248,160 @@ __device__ bool RewriteRow(const CommitParams& p, CommitShared& sh, int x) {
   return true;
 }
 
+// Rows of at most kWaveRow entries are merged and rewritten by one wave each
+// (no workgroup barriers), in a per-wave LDS table of kWaveSlots slots.
+constexpr int kWaveRow = kWave;
+constexpr int kWaveSlots = 2 * kWave;
+
+// UnionPair for lk + lr <= kWaveRow, on one wave: lane e holds entry e (keep
+// row, then remove row).  Returns false after a pool error.
+__device__ bool UnionPairWave(const CommitParams& p, int32_t* tk, int32_t* tc, int keep, int remove, int lk, int lr) {
+  const int lane = LaneId();
+  constexpr uint32_t mask = kWaveSlots - 1;
+  tk[lane] = -1;
+  tk[lane + kWave] = -1;
+  tc[lane] = 0;
+  tc[lane + kWave] = 0;
+  const int n = lk + lr;
+  const bool from_keep = lane < lk;
+  int m = -1, cnt = 0;
+  if (lane < n) {
+    const Entry en = from_keep ? p.pool[p.row_off[keep] + lane] : p.pool[p.row_off[remove] + (lane - lk)];
+    m = MapRep(p, en.nbr);
+    cnt = en.cnt;
+  }
+  WaveSync();
+  // keep->remove count joins the internal edges once (quotient_graph.cpp:637-647).
+  const long long internal = WaveSum64(lane < n && m == keep && from_keep ? static_cast<long long>(cnt) : 0LL);
+  bool fresh = false;
+  if (lane < n && m != keep) {
+    uint32_t h = HashInt(static_cast<uint32_t>(m)) & mask;
+    while (true) {
+      const int prev = atomicCAS(&tk[h], -1, m);
+      if (prev == -1 || prev == m) {
+        fresh = prev == -1;
+        break;
+      }
+      h = (h + 1) & mask;
+    }
+    atomicAdd(&tc[h], cnt);
+  }
+  const int distinct = __popcll(__ballot(fresh));
+  WaveSync();
+  unsigned long long dst = 0;
+  int fail = 0;
+  if (lane == 0) {
+    dst = p.row_off[keep];
+    if (distinct > p.row_cap[keep]) {
+      dst = atomicAdd(p.pool_top, static_cast<unsigned long long>(distinct));
+      if (dst + static_cast<unsigned long long>(distinct) > p.pool_cap) {
+        fail = 1;
+        atomicOr(p.err, kErrPool);
+      }
+    }
+  }
+  dst = ReadLane64(dst, 0);
+  if (__builtin_amdgcn_readfirstlane(fail)) return false;
+  Entry* out = p.pool + dst;
+  const unsigned long long below = (1ULL << lane) - 1ULL;
+  int base = 0;
+#pragma unroll
+  for (int half = 0; half < 2; ++half) {
+    const int sl = lane + half * kWave;
+    const int key = tk[sl];
+    const unsigned long long hm = __ballot(key >= 0);
+    if (key >= 0) {
+      Entry en;
+      en.nbr = key;
+      en.cnt = tc[sl];
+      out[base + __popcll(hm & below)] = en;
+    }
+    base += __popcll(hm);
+  }
+  if (lane == 0) {
+    if (dst != p.row_off[keep]) {
+      p.row_off[keep] = static_cast<uint32_t>(dst);
+      p.row_cap[keep] = distinct;
+    }
+    p.row_len[keep] = distinct;
+    p.size[keep] += p.size[remove];
+    p.internal[keep] += p.internal[remove] + static_cast<int32_t>(internal);
+    p.row_len[remove] = 0;
+    p.size[remove] = 0;
+    p.internal[remove] = 0;
+    p.active[remove] = 0;
+    atomicAdd(&p.ctr->merges, 1ULL);
+  }
+  WaveSync();
+  return true;
+}
+
+// RewriteRow for lx <= kWaveRow, on one wave: lane e holds entry e.  A
+// remove entry whose keep is also in the row folds into it and is deleted;
+// otherwise it is re-pointed to its keep (quotient_graph.cpp:696-712).
+__device__ void RewriteRowWave(const CommitParams& p, int32_t* tk, int32_t* tc, int x, int lx) {
+  const int lane = LaneId();
+  constexpr uint32_t mask = kWaveSlots - 1;
+  tk[lane] = -1;
+  tk[lane + kWave] = -1;
+  tc[lane] = 0;
+  tc[lane + kWave] = 0;
+  Entry* row = p.pool + p.row_off[x];
+  Entry en{-1, 0};
+  bool is_keep = false, is_rem = false;
+  if (lane < lx) {
+    en = row[lane];
+    is_keep = p.mark_keep[en.nbr] == p.epoch;
+    is_rem = !is_keep && p.mark_dest[en.nbr] == p.epoch;
+  }
+  WaveSync();
+  uint32_t hk = 0;
+  if (is_keep) {  // keep entries by neighbour (each neighbour once per row)
+    hk = HashInt(static_cast<uint32_t>(en.nbr)) & mask;
+    while (atomicCAS(&tk[hk], -1, en.nbr) != -1) hk = (hk + 1) & mask;
+  }
+  WaveSync();
+  bool del = false, changed = false;
+  if (is_rem) {
+    const int k = p.dest[en.nbr];
+    uint32_t h = HashInt(static_cast<uint32_t>(k)) & mask;
+    bool found = false;
+    while (true) {
+      const int key = tk[h];
+      if (key == k) {
+        found = true;
+        break;
+      }
+      if (key == -1) break;
+      h = (h + 1) & mask;
+    }
+    if (found) {
+      atomicAdd(&tc[h], en.cnt);
+      del = true;
+    } else {
+      en.nbr = k;
+      changed = true;
+    }
+  }
+  WaveSync();
+  if (is_keep) {
+    const int add = tc[hk];
+    if (add != 0) {
+      en.cnt += add;
+      changed = true;
+    }
+  }
+  const unsigned long long dm = __ballot(del);
+  if (dm == 0) {
+    if (changed) row[lane] = en;
+  } else {
+    const unsigned long long live = __ballot(lane < lx && !del);
+    if (lane < lx && !del) row[__popcll(live & ((1ULL << lane) - 1ULL))] = en;
+    if (lane == 0) p.row_len[x] = lx - __popcll(dm);
+  }
+  WaveSync();
+}
+
 // Exclusive prefix of the selection counts of `nslots` slots into off[0..nslots].
 __device__ void SlotPrefix(const int32_t* sel_count, int nslots, uint32_t* off, uint32_t* wave_tot) {
   const int per = (nslots + kThreads - 1) / kThreads;
@@ -304,22 +459,79 @@ __global__ __launch_bounds__(kThreads) void CommitKernel(CommitParams p) {
     npairs = sh.slot_off[nslots];
   }
   const uint32_t naff = static_cast<uint32_t>(*p.naff);
-  for (uint32_t item = blockIdx.x; item < npairs + naff; item += gridDim.x) {
-    if (item < npairs) {
-      int slot, keep, remove;
-      if (p.pairs != nullptr) {
-        const int2 pr = p.pairs[item];
-        keep = pr.x;
-        remove = pr.y;
+  const uint32_t total = npairs + naff;
+  // Chunks of one item per wave, items spread over the workgroups first
+  // (item = workgroup + grid * (wave + waves * chunk)), so large items land
+  // on different workgroups: rows of at most kWaveRow entries are done by
+  // their wave; larger items are listed and then done by the whole
+  // workgroup, one after another.
+  __shared__ int32_t wk_key[kThreads / kWave][kWaveSlots], wk_cnt[kThreads / kWave][kWaveSlots];
+  __shared__ uint32_t big[kThreads / kWave];
+  __shared__ int nbig, failed;
+  const int wave = WaveId(), lane = LaneId();
+  if (tid == 0) {
+    nbig = 0;
+    failed = 0;
+  }
+  __syncthreads();
+  const uint32_t step = gridDim.x * (kThreads / kWave);
+  for (uint32_t base = blockIdx.x; base < total; base += step) {
+    const uint32_t item = base + gridDim.x * static_cast<uint32_t>(wave);
+    if (item < total && !failed) {
+      if (item < npairs) {
+        int slot, keep, remove;
+        if (p.pairs != nullptr) {
+          const int2 pr = p.pairs[item];
+          keep = pr.x;
+          remove = pr.y;
+        } else {
+          WindowPair(p.sel, sh.slot_off, nslots, item, gbase, &slot, &keep, &remove);
+        }
+        const int lk = p.row_len[keep], lr = p.row_len[remove];
+        if (lk + lr <= kWaveRow) {
+          if (!UnionPairWave(p, wk_key[wave], wk_cnt[wave], keep, remove, lk, lr) && lane == 0) failed = 1;
+        } else if (lane == 0) {
+          big[atomicAdd(&nbig, 1)] = item;
+        }
       } else {
-        WindowPair(p.sel, sh.slot_off, nslots, item, gbase, &slot, &keep, &remove);
+        const int x = p.aff[item - npairs];
+        if (p.mark_dest[x] != p.epoch && p.mark_keep[x] != p.epoch) {
+          const int lx = p.row_len[x];
+          if (lx <= kWaveRow) {
+            RewriteRowWave(p, wk_key[wave], wk_cnt[wave], x, lx);
+          } else if (lane == 0) {
+            big[atomicAdd(&nbig, 1)] = item;
+          }
+        }
       }
-      if (!UnionPair(p, sh, keep, remove)) return;
-    } else {
-      const int x = p.aff[item - npairs];
-      if (p.mark_dest[x] == p.epoch || p.mark_keep[x] == p.epoch) continue;
-      if (!RewriteRow(p, sh, x)) return;
     }
+    __syncthreads();
+    const int nb = nbig;
+    for (int k = 0; k < nb && !failed; ++k) {
+      const uint32_t bi = big[k];
+      bool ok;
+      if (bi < npairs) {
+        int slot, keep, remove;
+        if (p.pairs != nullptr) {
+          const int2 pr = p.pairs[bi];
+          keep = pr.x;
+          remove = pr.y;
+        } else {
+          WindowPair(p.sel, sh.slot_off, nslots, bi, gbase, &slot, &keep, &remove);
+        }
+        ok = UnionPair(p, sh, keep, remove);
+      } else {
+        ok = RewriteRow(p, sh, p.aff[bi - npairs]);
+      }
+      if (!ok) {
+        if (tid == 0) failed = 1;
+        __syncthreads();
+      }
+    }
+    __syncthreads();
+    if (tid == 0) nbig = 0;
+    __syncthreads();
+    if (failed) return;
   }
 }
 
