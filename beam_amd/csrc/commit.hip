@@ -588,11 +588,16 @@ __device__ __forceinline__ int WavePairEntry(const RoundParams& p, const WavePai
   return *in_remove ? p.q.pool[orr + idx - lk].nbr : p.q.pool[ok + idx].nbr;
 }
 
-// Records, per node, the first horizon block whose selected pairs touch it.
+// j = the first horizon block (after the cursor's) with a member adjacent to
+// a keep or remove selected by an earlier block of the horizon; blocks
+// cursor .. cursor + j - 1 commit.  Every row entry x of a selected pair of
+// block b lowers j to x's block when that block is later than b (and x's
+// group has two or more members).  j starts at the horizon length (set with
+// the horizon, by MarkRound of the round before or the iteration's first
+// round).
 __global__ __launch_bounds__(kThreads) void TouchRound(RoundParams p) {
   const int cursor = p.cur->cursor, end = p.cur->end;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    p.cur->j = end - cursor;
     p.next->ntodo = 0;
     p.next->steal = 0;
     p.next->nsplit = p.next->npair_items = p.next->nrow_items = p.next->steal_pairs = p.next->steal_rows = 0;
@@ -608,8 +613,7 @@ __global__ __launch_bounds__(kThreads) void TouchRound(RoundParams p) {
   for (int gi = first + static_cast<int>(blockIdx.x) * (kThreads / kWave) + WaveId(); gi < last; gi += waves) {
     const int cnt = p.glen[gi] >= 2 ? p.sel_count[gi] : 0;
     if (cnt == 0) continue;
-    const unsigned long long val = (static_cast<unsigned long long>(p.epoch) << 16) |
-                                   static_cast<unsigned long long>(0xffff - (gi / p.block_size - cursor));
+    const int b = gi / p.block_size - cursor;
     const int32_t* sp = p.sel + p.gstart[gi];
     for (int k0 = 0; k0 < cnt; k0 += kWave) {
       const WavePairs w = LoadWavePairs(p, sp, k0 + lane, cnt);
@@ -619,32 +623,17 @@ __global__ __launch_bounds__(kThreads) void TouchRound(RoundParams p) {
         int owner;
         const uint32_t e = min(e0 + static_cast<uint32_t>(lane), total - 1);
         const int x = WavePairEntry(p, w, e, &in_remove, &owner);
-        if (e0 + static_cast<uint32_t>(lane) < total) atomicMax(&p.touch[x], val);
+        int hit = 0x7fffffff;
+        if (e0 + static_cast<uint32_t>(lane) < total) {
+          const int gx = p.node_group[x];
+          const int bx = gx / p.block_size - cursor;
+          if (bx > b && gx < last && p.glen[gx] >= 2) hit = bx;
+        }
+#pragma unroll
+        for (int off = kWave / 2; off > 0; off >>= 1) hit = min(hit, __shfl_xor(hit, off, kWave));
+        if (lane == 0 && hit != 0x7fffffff) atomicMin(&p.cur->j, hit);
       }
     }
-  }
-}
-
-// j = the first horizon block (after the cursor's) with a member touched by
-// an earlier block of the horizon; blocks cursor .. cursor + j - 1 commit.
-// One wave per group, its lanes over the members.
-__global__ __launch_bounds__(kThreads) void FindCommitBlocks(RoundParams p) {
-  const int cursor = p.cur->cursor, end = p.cur->end;
-  const int first = (cursor + 1) * p.block_size;
-  const int last = min(p.ngroups, end * p.block_size);
-  const int lane = LaneId();
-  const int waves = static_cast<int>(gridDim.x) * (kThreads / kWave);
-  for (int gi = first + static_cast<int>(blockIdx.x) * (kThreads / kWave) + WaveId(); gi < last; gi += waves) {
-    const int len = p.glen[gi];
-    if (len < 2) continue;
-    const int rel = gi / p.block_size - cursor;
-    const int32_t* members = p.order + p.gstart[gi];
-    bool hit = false;
-    for (int k = lane; k < len; k += kWave) {
-      const unsigned long long t = p.touch[members[k]];
-      hit |= static_cast<int>(t >> 16) == p.epoch && 0xffff - static_cast<int>(t & 0xffff) < rel;
-    }
-    if (__ballot(hit) != 0 && lane == 0) atomicMin(&p.cur->j, rel);
   }
 }
 
@@ -663,6 +652,7 @@ __global__ __launch_bounds__(kThreads) void MarkRound(RoundParams p) {
   if (blockIdx.x == 0 && tid == 0) {
     p.next->cursor = ncur;
     p.next->end = nend;
+    p.next->j = nend - ncur;
     p.commit_out[0] = clast - first;
     p.commit_out[1] = j;
     p.commit_out[2] = first;

@@ -254,7 +254,7 @@ __global__ void InitRound(RoundState* rs, int end, int32_t* naff, unsigned long 
   if (threadIdx.x != 0) return;
   rs->cursor = 0;
   rs->end = end;
-  rs->j = 0;
+  rs->j = end;
   rs->ntodo = 0;
   rs->steal = 0;
   rs->nsplit = rs->npair_items = rs->nrow_items = rs->steal_pairs = rs->steal_rows = 0;
@@ -458,7 +458,6 @@ struct Engine {
   int32_t* sel = nullptr;
   int32_t* sel_count = nullptr;
   long long* slot_stats = nullptr;
-  unsigned long long* touch = nullptr;
   int32_t* commit_out = nullptr;  // [0] committed slots, [1] committed blocks, [2] first group, [3] pairs
   int2* commit_pairs = nullptr;   // [n / 2 + 1] a round's committed pairs
   int32_t commit_host[2] = {0, 0};
@@ -580,7 +579,7 @@ Engine::~Engine() {
                   open_rep, mh,       recs[0],  recs[1],   flag,     excl,       rid,       run_start, run_open,
                   run_seg,  elem_open, new_idx, gflag,     gstart,   glen,       d_total,   d_maxlen, sel,
                   sel_count, mark_dest, dest,   mark_keep, mark_row, aff,        naff,      scratch_a, scratch_u,
-                  slot_stats, touch,  commit_out, commit_pairs, node_group, gmark,     todo[0],   todo[1], rs, split, pair_items, row_items, score_items, round_ctr, samples, sample_target, sample_key,
+                  slot_stats, commit_out, commit_pairs, node_group, gmark,     todo[0],   todo[1], rs, split, pair_items, row_items, score_items, round_ctr, samples, sample_target, sample_key,
                   sample_count, audit_out, audit_list, audit_rs,
                   tops,     ctr,      err,      enc,        trace_buf};
   for (void* p : ptrs)
@@ -658,7 +657,6 @@ void Engine::Create(const int64_t* h_row_ptr, const int32_t* h_col, int n_, long
   sel = DevAlloc<int32_t>(nsel);
   sel_count = DevAlloc<int32_t>(nslot);
   slot_stats = DevAlloc<long long>(nslot * kSlotStats);
-  touch = DevAlloc<unsigned long long>(n);
   commit_out = DevAlloc<int32_t>(4);
   commit_pairs = DevAlloc<int2>(static_cast<size_t>(n) / 2 + 1);
   node_group = DevAlloc<int32_t>(n);
@@ -731,7 +729,6 @@ void Engine::Reset() {
   HIP_CHECK(hipMemsetAsync(tops, 0, 2 * sizeof(unsigned long long), stream));
   HIP_CHECK(hipMemsetAsync(ctr, 0, sizeof(Counters) * (kMaxIterations + 1), stream));
   HIP_CHECK(hipMemsetAsync(err, 0, sizeof(int), stream));
-  HIP_CHECK(hipMemsetAsync(touch, 0, sizeof(unsigned long long) * n, stream));
   epoch = 0;
   windows_run = 0;
   blocks_run = 0;
@@ -1015,7 +1012,6 @@ void Engine::LaunchRound(int G, int nblocks, int block_size, double theta, Count
   rp.sel = sel;
   rp.sel_count = sel_count;
   rp.slot_stats = slot_stats;
-  rp.touch = touch;
   rp.gmark = gmark;
   rp.todo_next = todo[(epoch + 1) & 1];
   rp.mark_dest = mark_dest;
@@ -1035,7 +1031,6 @@ void Engine::LaunchRound(int G, int nblocks, int block_size, double theta, Count
   const int slots = horizon_blocks * block_size;
   const int wave_grid = (slots + 7) / 8;  // one wave per group, 8 waves per workgroup
   TouchRound<<<wave_grid, 512, 0, stream>>>(rp);
-  FindCommitBlocks<<<wave_grid, 512, 0, stream>>>(rp);
   MarkRound<<<wave_grid, 512, 0, stream>>>(rp);
   Commit(c, true);
   ++windows_run;
@@ -1242,7 +1237,7 @@ void Engine::Iterate(int iter) {
   const int nblocks = (G + bs - 1) / bs;
   if (nblocks > 0) {
     // Rounds: the horizon spans at most kMaxWindowSlots groups (the commit
-    // kernel's slot prefix lives in LDS) and 0xffff blocks (touch encoding).
+    // kernel's window-mode slot prefix lives in LDS).
     horizon_blocks = std::max(1, std::min(64, kMaxWindowSlots / bs));
     NodeGroups<<<Blocks(G, 256), 256, 0, stream>>>(G, gstart, glen, order, node_group);
     FillU32<<<Blocks(G, 256), 256, 0, stream>>>(reinterpret_cast<uint32_t*>(sel_count), G, 0U);

@@ -67,7 +67,7 @@ struct QuotientView {
 struct RoundState {
   int cursor;  // first uncommitted block of the iteration
   int end;     // horizon end (exclusive, blocks)
-  int j;       // blocks this round commits
+  int j;       // blocks this round commits (the horizon length until lowered)
   int ntodo;   // groups queued for scoring in this round
   int steal;   // scoring work counter
   // Groups of kSplitG+ members are split over workgroups (see kSplitGate): the group
@@ -193,7 +193,6 @@ struct RoundParams {
   const int32_t* sel;
   const int32_t* sel_count;
   const long long* slot_stats;
-  unsigned long long* touch;  // (epoch << 16) | (0xffff - first touching block)
   int32_t* gmark;
   int32_t* todo_next;
   int32_t* mark_dest;
@@ -255,7 +254,6 @@ __global__ void SplitRowsRankerKernel(MergeParams p);
 __global__ void SplitScoreKernel(MergeParams p);         // exact-scoring chunks, then the group's selection
 __global__ void CommitKernel(CommitParams p);
 __global__ void TouchRound(RoundParams p);
-__global__ void FindCommitBlocks(RoundParams p);
 __global__ void MarkRound(RoundParams p);
 __global__ void SumGroupStats(const long long* slot_stats, const int32_t* glen, int ngroups, Counters* c);
 
