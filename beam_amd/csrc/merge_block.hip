@@ -1151,21 +1151,31 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
 
     stamp(7);
     if (small) {
-      // Phase 4: bucket offsets, scatter, pairs per bucket.
-      ScanInPlace(A.th_cnt, static_cast<int>(TC), wave_tot);
+      // Phase 4: bucket offsets and pairs per bucket in one scan (entries in
+      // the low 12 bits -- at most 2,048 -- and pairs above them -- at most
+      // 32 full buckets of 64, 64,512 < 2^17), then the scatter.
+#pragma unroll 8
+      for (uint32_t k = tid; k < TC; k += kThreads) {
+        const uint32_t n = A.th_cnt[k];
+        A.pair_pref[k] = ((n >= 2 ? n * (n - 1) / 2 : 0U) << 12) | n;
+      }
+      __syncthreads();
+      const uint32_t packed = ScanInPlace(A.pair_pref, static_cast<int>(TC), wave_tot);
+#pragma unroll 8
+      for (uint32_t k = tid; k <= TC; k += kThreads) {
+        const uint32_t v = A.pair_pref[k];
+        A.th_cnt[k] = v & 0xfffU;
+        A.pair_pref[k] = v >> 12;
+      }
+      __syncthreads();
+      PT = Uniform(packed >> 12);
 #pragma unroll 4
       for (uint32_t e = tid; e < E; e += kThreads) {
         const int sl = A.ent_slot[e];
         if (sl < 0) continue;
         s_bucket[A.th_cnt[sl] + static_cast<uint32_t>(s_ent_rank[e])] = static_cast<int16_t>(e);
       }
-#pragma unroll 8
-      for (uint32_t k = tid; k < TC; k += kThreads) {
-        const uint32_t n = A.th_cnt[k + 1] - A.th_cnt[k];
-        A.pair_pref[k] = n >= 2 ? n * (n - 1) / 2 : 0;
-      }
       __syncthreads();
-      PT = Uniform(ScanInPlace(A.pair_pref, static_cast<int>(TC), wave_tot));
 
       // Phase 5: bucket pair expansion (sweg.cpp:1863-1903): for two members
       // sharing target t, delta = EC(a,t,ea) + EC(b,t,eb) - EC(a+b,t,ea+eb),
