@@ -625,10 +625,13 @@ __device__ __forceinline__ bool SelectGroup(const MergeParams& p, int g, int slo
     const long long gn = A.cand_gain[pi], bf = A.cand_before[pi];
     return gn > 0 && bf > 0 && !(static_cast<double>(gn) / static_cast<double>(bf) < p.theta);
   };
+  int npos = 0;  // positive gains, summed per wave before one LDS atomic
   for (uint32_t pi = tid; pi < C; pi += kThreads) {
-    if (A.cand_gain[pi] > 0) atomicAdd(&s_counts[0], 1U);
+    npos += A.cand_gain[pi] > 0 ? 1 : 0;
     A.pass[pi] = passes(pi) ? 1U : 0U;
   }
+  npos = WaveSum32(npos);
+  if (LaneId() == 0 && npos != 0) atomicAdd(&s_counts[0], static_cast<uint32_t>(npos));
   if (tid == 0) {
     s_nchosen = 0;
     s_gain = 0;
@@ -651,7 +654,10 @@ __device__ __forceinline__ bool SelectGroup(const MergeParams& p, int g, int slo
                          static_cast<unsigned long long>(A.cand_b[pi]);
     }
   }
-  for (int i = tid; i < g; i += kThreads) atomicAdd(&s_entries, static_cast<unsigned long long>(m_len[i]));
+  long long ent = 0;
+  for (int i = tid; i < g; i += kThreads) ent += m_len[i];
+  ent = WaveSum64(ent);
+  if (LaneId() == 0 && ent != 0) atomicAdd(&s_entries, static_cast<unsigned long long>(ent));
   __syncthreads();
 
   // Phase 11: greedy endpoint-disjoint selection in key order
