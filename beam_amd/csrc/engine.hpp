@@ -89,7 +89,7 @@ struct RoundState {
 
 // Groups of at least kSplitG members (the default threshold) are split, in
 // iterations whose largest group has at least max(threshold, kSplitGate)
-// members: in iterations with only medium groups the two extra launches per
+// members: in iterations with only medium groups the three extra launches per
 // round cost more than they save.
 constexpr int kSplitG = 65;
 constexpr int kSplitGate = 200;
