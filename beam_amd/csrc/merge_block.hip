@@ -1946,35 +1946,6 @@ __device__ __forceinline__ void ScoreGroup(const MergeParams& p, int gidx, int s
       }
       __syncthreads();
     }
-    if (mode == kModeBack && p.score_items != nullptr && C > 0) {
-      // Queue the candidates in chunks for the scoring kernel, with what
-      // scoring needs per member; without room they are scored here.
-      const int sc = static_cast<int>((C + kScoreChunk - 1) / kScoreChunk);
-      if (tid == 0) {
-        const int sb = atomicAdd(&p.round->nscore_items, sc);
-        s_flag = sb + sc <= p.item_cap ? 1 : 0;
-        s_pb = sb;
-      }
-      __syncthreads();
-      const int sb = s_pb;
-      const int ok = s_flag;
-      __syncthreads();
-      for (int c = tid; c < sc && sb + c < p.item_cap; c += kThreads)
-        p.score_items[sb + c] = make_int2(ok ? static_cast<int>(rec - p.split) : -1, c);
-      if (ok) {
-        for (int i = tid; i < g; i += kThreads) {
-          A.mlen[i] = m_len[i];
-          A.mint[i] = m_int[i];
-          A.meoff[i] = m_eoff[i];
-        }
-        if (tid == 0) {
-          rec->C = C;
-          rec->score_chunks = sc;
-          rec->score_done = 0;
-        }
-        return;
-      }
-    }
     for (uint32_t pi = tid; pi < C; pi += kThreads) {
       const int a = A.cand_a[pi], b = A.cand_b[pi];
       const int64_t sa = m_size[a], sb = m_size[b], sw = sa + sb;
