@@ -125,11 +125,16 @@ __global__ void ResetMinhash(int K, const uint32_t* open_pos, const int32_t* ord
 
 // Min over each open supernode's members of the node shingle: the minimum
 // HashRank64 over the node and its original neighbours (sweg.cpp:754-784).
+// Nodes of more than kHeavyDegree neighbours are left to NodeShinglesHeavy
+// (one thread looping over a hub's row would set the launch's duration).
+constexpr int64_t kHeavyDegree = 1024;
+
 __global__ void NodeShingles(int n, const int64_t* row_ptr, const int32_t* col, const int32_t* label,
                              const unsigned char* open_rep, unsigned long long* mh, uint64_t seed, int dim,
                              int iter) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= n) return;
+  if (row_ptr[v + 1] - row_ptr[v] > kHeavyDegree) return;
   const int r = label[v];
   if (!open_rep[r]) return;
   uint64_t best = HashRank64(seed, v, dim, iter);
@@ -138,6 +143,35 @@ __global__ void NodeShingles(int n, const int64_t* row_ptr, const int32_t* col, 
     best = h < best ? h : best;
   }
   atomicMin(&mh[r], static_cast<unsigned long long>(best));
+}
+
+// NodeShingles for the heavy nodes, one workgroup of 256 threads per node: a
+// minimum over the row's slices, reduced across the workgroup.
+__global__ __launch_bounds__(256) void NodeShinglesHeavy(const int32_t* heavy, const int64_t* row_ptr,
+                                                         const int32_t* col, const int32_t* label,
+                                                         const unsigned char* open_rep, unsigned long long* mh,
+                                                         uint64_t seed, int dim, int iter) {
+  __shared__ unsigned long long wmin[256 / kWave];
+  const int v = heavy[blockIdx.x];
+  const int r = label[v];
+  if (!open_rep[r]) return;
+  uint64_t best = threadIdx.x == 0 ? HashRank64(seed, v, dim, iter) : ~0ULL;
+  for (int64_t k = row_ptr[v] + threadIdx.x; k < row_ptr[v + 1]; k += 256) {
+    const uint64_t h = HashRank64(seed, col[k], dim, iter);
+    best = h < best ? h : best;
+  }
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    const uint64_t o = (static_cast<uint64_t>(static_cast<uint32_t>(__shfl_xor(static_cast<int>(best >> 32), off, kWave))) << 32) |
+                       static_cast<uint32_t>(__shfl_xor(static_cast<int>(static_cast<uint32_t>(best)), off, kWave));
+    best = o < best ? o : best;
+  }
+  if (LaneId() == 0) wmin[WaveId()] = best;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < 256 / kWave; ++w) best = wmin[w] < best ? wmin[w] : best;
+    atomicMin(&mh[r], static_cast<unsigned long long>(best));
+  }
 }
 
 __global__ void MakeKeys(int K, const uint32_t* open_pos, const uint32_t* seg, const int32_t* order,
@@ -433,6 +467,8 @@ struct Engine {
 
   // Divide
   int32_t* order = nullptr;
+  int32_t* heavy = nullptr;  // nodes of degree above kHeavyDegree
+  int nheavy = 0;
   uint32_t* open_pos[2] = {nullptr, nullptr};
   uint32_t* seg[2] = {nullptr, nullptr};
   unsigned char* open_rep = nullptr;
@@ -575,7 +611,7 @@ struct Engine {
 Engine::~Engine() {
   if (stream) (void)hipStreamSynchronize(stream);
   void* ptrs[] = {row_ptr,  col,      label,    parent,    merge_seq, active,   pool[0],    pool[1],   pool_top, row_off,
-                  row_len,  row_cap,  size,     internal,  order,    open_pos[0], open_pos[1], seg[0], seg[1],
+                  row_len,  row_cap,  size,     internal,  order,    heavy,    open_pos[0], open_pos[1], seg[0], seg[1],
                   open_rep, mh,       recs[0],  recs[1],   flag,     excl,       rid,       run_start, run_open,
                   run_seg,  elem_open, new_idx, gflag,     gstart,   glen,       d_total,   d_maxlen, sel,
                   sel_count, mark_dest, dest,   mark_keep, mark_row, aff,        naff,      scratch_a, scratch_u,
@@ -609,6 +645,17 @@ void Engine::Create(const int64_t* h_row_ptr, const int32_t* h_col, int n_, long
   col = DevAlloc<int32_t>(static_cast<size_t>(arcs));
   HIP_CHECK(hipMemcpyAsync(row_ptr, h_row_ptr, (static_cast<size_t>(n) + 1) * sizeof(int64_t), hipMemcpyHostToDevice, stream));
   HIP_CHECK(hipMemcpyAsync(col, h_col, static_cast<size_t>(arcs) * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+  {
+    std::vector<int32_t> h_heavy;
+    for (int v = 0; v < n; ++v)
+      if (h_row_ptr[v + 1] - h_row_ptr[v] > kHeavyDegree) h_heavy.push_back(v);
+    nheavy = static_cast<int>(h_heavy.size());
+    if (nheavy > 0) {
+      heavy = DevAlloc<int32_t>(nheavy);
+      HIP_CHECK(hipMemcpyAsync(heavy, h_heavy.data(), h_heavy.size() * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+      HIP_CHECK(hipStreamSynchronize(stream));
+    }
+  }
   label = DevAlloc<int32_t>(n);
   parent = DevAlloc<int32_t>(n);
   merge_seq = DevAlloc<int32_t>(n);
@@ -839,6 +886,9 @@ int Engine::Divide(int iter) {
     const int dim = dims[static_cast<size_t>(depth)];
     ResetMinhash<<<Blocks(K, 256), 256, 0, stream>>>(K, open_pos[cur], order, mh);
     NodeShingles<<<Blocks(n, 256), 256, 0, stream>>>(n, row_ptr, col, label, open_rep, mh, opt.seed, dim, iter);
+    if (nheavy > 0) {
+      NodeShinglesHeavy<<<nheavy, 256, 0, stream>>>(heavy, row_ptr, col, label, open_rep, mh, opt.seed, dim, iter);
+    }
     MakeKeys<<<Blocks(K, 256), 256, 0, stream>>>(K, open_pos[cur], seg[cur], order, mh, recs[0]);
     SortRec* sorted = SegmentedKeySort(recs[0], recs[1], K, sort, stream);
     WriteBack<<<Blocks(K, 256), 256, 0, stream>>>(K, sorted, open_pos[cur], order);

@@ -217,6 +217,22 @@ def test_divide_matches_oracle(graph_path, merges):
         np.testing.assert_array_equal(e_order, o_order)
 
 
+def test_divide_with_hub_nodes_matches_oracle(graph_path):
+    """yo holds 362 nodes of degree above 1024, whose shingles take the
+    workgroup-per-node kernel; the groups must not change."""
+    print("case: divide yo hubs", flush=True)
+    oracle = Oracle(graph_path("yo"), seed=3)
+    engine = make_engine(oracle, seed=3)
+    row_ptr, _ = oracle.csr()
+    assert int((np.diff(row_ptr) > 1024).sum()) > 0
+    for it in (1, 4):
+        o_order, o_starts, o_len = oracle.divide(it)
+        e_order, e_starts, e_len = engine.divide(it)
+        np.testing.assert_array_equal(e_starts, o_starts)
+        np.testing.assert_array_equal(e_len, o_len)
+        np.testing.assert_array_equal(e_order, o_order)
+
+
 def _golden(name):
     with open(os.path.join(GOLDEN, f"{name}_summary.json")) as f:
         return json.load(f)
