@@ -125,8 +125,9 @@ __global__ void ResetMinhash(int K, const uint32_t* open_pos, const int32_t* ord
 
 // Min over each open supernode's members of the node shingle: the minimum
 // HashRank64 over the node and its original neighbours (sweg.cpp:754-784).
-// Nodes of more than kHeavyDegree neighbours are left to NodeShinglesHeavy
-// (one thread looping over a hub's row would set the launch's duration).
+// Nodes of more than kMidDegree neighbours are left to NodeShinglesWide (one
+// thread looping over a long row would set the launch's duration).
+constexpr int64_t kMidDegree = 64;
 constexpr int64_t kHeavyDegree = 1024;
 
 __global__ void NodeShingles(int n, const int64_t* row_ptr, const int32_t* col, const int32_t* label,
@@ -134,7 +135,7 @@ __global__ void NodeShingles(int n, const int64_t* row_ptr, const int32_t* col, 
                              int iter) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= n) return;
-  if (row_ptr[v + 1] - row_ptr[v] > kHeavyDegree) return;
+  if (row_ptr[v + 1] - row_ptr[v] > kMidDegree) return;
   const int r = label[v];
   if (!open_rep[r]) return;
   uint64_t best = HashRank64(seed, v, dim, iter);
@@ -145,18 +146,24 @@ __global__ void NodeShingles(int n, const int64_t* row_ptr, const int32_t* col, 
   atomicMin(&mh[r], static_cast<unsigned long long>(best));
 }
 
-// NodeShingles for the heavy nodes, one workgroup of 256 threads per node: a
-// minimum over the row's slices, reduced across the workgroup.
-__global__ __launch_bounds__(256) void NodeShinglesHeavy(const int32_t* heavy, const int64_t* row_ptr,
-                                                         const int32_t* col, const int32_t* label,
-                                                         const unsigned char* open_rep, unsigned long long* mh,
-                                                         uint64_t seed, int dim, int iter) {
+// NodeShingles for the long rows, kSpan threads per node (a wave per node of
+// degree kMidDegree + 1 .. kHeavyDegree, a 256-thread workgroup per hub): a
+// minimum over the row's slices, reduced across the span.  wide[0 .. nheavy)
+// are the hubs, wide[nheavy ..) the mid-degree nodes.
+template <int kSpan>
+__global__ __launch_bounds__(256) void NodeShinglesWide(const int32_t* wide, int count, const int64_t* row_ptr,
+                                                        const int32_t* col, const int32_t* label,
+                                                        const unsigned char* open_rep, unsigned long long* mh,
+                                                        uint64_t seed, int dim, int iter) {
   __shared__ unsigned long long wmin[256 / kWave];
-  const int v = heavy[blockIdx.x];
+  const int i = static_cast<int>(blockIdx.x) * (256 / kSpan) + static_cast<int>(threadIdx.x) / kSpan;
+  const int t = static_cast<int>(threadIdx.x) % kSpan;
+  if (i >= count) return;
+  const int v = wide[i];
   const int r = label[v];
   if (!open_rep[r]) return;
-  uint64_t best = threadIdx.x == 0 ? HashRank64(seed, v, dim, iter) : ~0ULL;
-  for (int64_t k = row_ptr[v] + threadIdx.x; k < row_ptr[v + 1]; k += 256) {
+  uint64_t best = t == 0 ? HashRank64(seed, v, dim, iter) : ~0ULL;
+  for (int64_t k = row_ptr[v] + t; k < row_ptr[v + 1]; k += kSpan) {
     const uint64_t h = HashRank64(seed, col[k], dim, iter);
     best = h < best ? h : best;
   }
@@ -166,11 +173,16 @@ __global__ __launch_bounds__(256) void NodeShinglesHeavy(const int32_t* heavy, c
                        static_cast<uint32_t>(__shfl_xor(static_cast<int>(static_cast<uint32_t>(best)), off, kWave));
     best = o < best ? o : best;
   }
-  if (LaneId() == 0) wmin[WaveId()] = best;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int w = 1; w < 256 / kWave; ++w) best = wmin[w] < best ? wmin[w] : best;
-    atomicMin(&mh[r], static_cast<unsigned long long>(best));
+  if constexpr (kSpan == kWave) {
+    if (t == 0) atomicMin(&mh[r], static_cast<unsigned long long>(best));
+  } else {
+    static_assert(kSpan == 256, "a span is one wave or the whole workgroup");
+    if (LaneId() == 0) wmin[WaveId()] = best;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int w = 1; w < 256 / kWave; ++w) best = wmin[w] < best ? wmin[w] : best;
+      atomicMin(&mh[r], static_cast<unsigned long long>(best));
+    }
   }
 }
 
@@ -467,8 +479,8 @@ struct Engine {
 
   // Divide
   int32_t* order = nullptr;
-  int32_t* heavy = nullptr;  // nodes of degree above kHeavyDegree
-  int nheavy = 0;
+  int32_t* wide = nullptr;  // nodes of degree above kHeavyDegree, then those above kMidDegree
+  int nheavy = 0, nmid = 0;
   uint32_t* open_pos[2] = {nullptr, nullptr};
   uint32_t* seg[2] = {nullptr, nullptr};
   unsigned char* open_rep = nullptr;
@@ -611,7 +623,7 @@ struct Engine {
 Engine::~Engine() {
   if (stream) (void)hipStreamSynchronize(stream);
   void* ptrs[] = {row_ptr,  col,      label,    parent,    merge_seq, active,   pool[0],    pool[1],   pool_top, row_off,
-                  row_len,  row_cap,  size,     internal,  order,    heavy,    open_pos[0], open_pos[1], seg[0], seg[1],
+                  row_len,  row_cap,  size,     internal,  order,    wide,     open_pos[0], open_pos[1], seg[0], seg[1],
                   open_rep, mh,       recs[0],  recs[1],   flag,     excl,       rid,       run_start, run_open,
                   run_seg,  elem_open, new_idx, gflag,     gstart,   glen,       d_total,   d_maxlen, sel,
                   sel_count, mark_dest, dest,   mark_keep, mark_row, aff,        naff,      scratch_a, scratch_u,
@@ -646,13 +658,18 @@ void Engine::Create(const int64_t* h_row_ptr, const int32_t* h_col, int n_, long
   HIP_CHECK(hipMemcpyAsync(row_ptr, h_row_ptr, (static_cast<size_t>(n) + 1) * sizeof(int64_t), hipMemcpyHostToDevice, stream));
   HIP_CHECK(hipMemcpyAsync(col, h_col, static_cast<size_t>(arcs) * sizeof(int32_t), hipMemcpyHostToDevice, stream));
   {
-    std::vector<int32_t> h_heavy;
-    for (int v = 0; v < n; ++v)
-      if (h_row_ptr[v + 1] - h_row_ptr[v] > kHeavyDegree) h_heavy.push_back(v);
-    nheavy = static_cast<int>(h_heavy.size());
-    if (nheavy > 0) {
-      heavy = DevAlloc<int32_t>(nheavy);
-      HIP_CHECK(hipMemcpyAsync(heavy, h_heavy.data(), h_heavy.size() * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+    std::vector<int32_t> h_wide, h_mid;
+    for (int v = 0; v < n; ++v) {
+      const int64_t deg = h_row_ptr[v + 1] - h_row_ptr[v];
+      if (deg > kHeavyDegree) h_wide.push_back(v);
+      else if (deg > kMidDegree) h_mid.push_back(v);
+    }
+    nheavy = static_cast<int>(h_wide.size());
+    nmid = static_cast<int>(h_mid.size());
+    h_wide.insert(h_wide.end(), h_mid.begin(), h_mid.end());
+    if (!h_wide.empty()) {
+      wide = DevAlloc<int32_t>(h_wide.size());
+      HIP_CHECK(hipMemcpyAsync(wide, h_wide.data(), h_wide.size() * sizeof(int32_t), hipMemcpyHostToDevice, stream));
       HIP_CHECK(hipStreamSynchronize(stream));
     }
   }
@@ -887,7 +904,12 @@ int Engine::Divide(int iter) {
     ResetMinhash<<<Blocks(K, 256), 256, 0, stream>>>(K, open_pos[cur], order, mh);
     NodeShingles<<<Blocks(n, 256), 256, 0, stream>>>(n, row_ptr, col, label, open_rep, mh, opt.seed, dim, iter);
     if (nheavy > 0) {
-      NodeShinglesHeavy<<<nheavy, 256, 0, stream>>>(heavy, row_ptr, col, label, open_rep, mh, opt.seed, dim, iter);
+      NodeShinglesWide<256><<<nheavy, 256, 0, stream>>>(wide, nheavy, row_ptr, col, label, open_rep, mh, opt.seed,
+                                                       dim, iter);
+    }
+    if (nmid > 0) {
+      NodeShinglesWide<kWave><<<(nmid + 3) / 4, 256, 0, stream>>>(wide + nheavy, nmid, row_ptr, col, label, open_rep,
+                                                                  mh, opt.seed, dim, iter);
     }
     MakeKeys<<<Blocks(K, 256), 256, 0, stream>>>(K, open_pos[cur], seg[cur], order, mh, recs[0]);
     SortRec* sorted = SegmentedKeySort(recs[0], recs[1], K, sort, stream);
