@@ -21,7 +21,11 @@ namespace {
 thread_local std::string g_error;
 
 constexpr int kMaxIterations = 4096;
+// Commit workgroups per launch: graphs under kCommitSmallArcs arcs commit few
+// pairs per round, where half the grid starts sooner and loses nothing.
 constexpr int kCommitGrid = 512;
+constexpr int kCommitGridSmall = 256;
+constexpr long long kCommitSmallArcs = 1LL << 22;
 constexpr int kSplitCap = 4096;       // split groups per round
 constexpr int kSplitItems = 1 << 18;  // pair chunks (and row chunks) per round
 // Learned ranker constants.
@@ -1156,7 +1160,7 @@ void Engine::Commit(Counters* c, bool round) {
   cp.reset_top = &tops[0];
   cp.ctr = c;
   cp.err = err;
-  CommitKernel<<<kCommitGrid, 512, 0, stream>>>(cp);
+  CommitKernel<<<arcs < kCommitSmallArcs ? kCommitGridSmall : kCommitGrid, 512, 0, stream>>>(cp);
 }
 
 void Engine::EnsurePoolRoom() {
